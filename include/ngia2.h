@@ -1,0 +1,215 @@
+/*
+ * ngia2.h -- C-ABI of the MI355X-native nGIA2 clustering hot path.
+ *
+ * The drop-in boundary.  Plain pointers and sizes, int status codes (0 = ok,
+ * negative = error, message via ngia_last_error), no C++ exceptions and no
+ * torch types cross it.  Every entry point that has a counterpart in the
+ * reference names it (paths relative to the nGIA2 source tree).
+ *
+ * Ownership: the caller owns every host buffer it passes; a context owns the
+ * device memory and the HIP stream it creates; arrays returned through a
+ * pointer-to-pointer are released with ngia_free.  One context per host
+ * thread; calls on one context are serialised by the caller.
+ *
+ * Integer results (signatures, representative rows, candidate pairs, scores,
+ * edges) are bit-identical to the reference algorithm on the same inputs.
+ */
+#ifndef NGIA2_H
+#define NGIA2_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NGIA_BANDS 64        /* GROUP_BANDS, makedb.hpp:45 */
+#define NGIA_MAX_LEN 0xFFFEu /* reads >= 0xFFFF are dropped, makedb.hpp:118 */
+
+/* status codes */
+#define NGIA_OK 0
+#define NGIA_ERR_ARG (-1)    /* bad argument (size, range, null pointer) */
+#define NGIA_ERR_HIP (-2)    /* HIP runtime / kernel failure */
+#define NGIA_ERR_NOMEM (-3)  /* device or host allocation failed */
+#define NGIA_ERR_IO (-4)     /* file read/write failure */
+#define NGIA_ERR_STATE (-5)  /* call out of order (e.g. graph before reps) */
+#define NGIA_ERR_NODEV (-6)  /* no HIP device */
+
+typedef struct ngia_ctx ngia_ctx;
+typedef struct ngia_db ngia_db;
+typedef struct ngia_serve ngia_serve;
+
+/* One graph edge, the reference's std::tuple<int,int,float> (buildGraph.hpp:256). */
+typedef struct {
+  int32_t u;   /* representative index (u < v) */
+  int32_t v;   /* target index */
+  float w;     /* (float)score / 100.0f, score in 1..100 */
+} ngia_edge;
+
+/* Graph-build modes (flags of ngia_db_graph / ngia_build_graph). */
+#define NGIA_GRAPH_PLAIN 0u    /* one align launch + host sync per band (buildGraph.hpp:274-350) */
+#define NGIA_GRAPH_BATCHED 1u  /* cross-band dedupe, length-binned kernels, one hipGraph */
+
+typedef struct {
+  uint64_t total_pairs;     /* candidates summed over bands, duplicates included ("compare pairs", buildGraph.hpp:353) */
+  uint64_t distinct_pairs;  /* distinct (rep,tar) pairs actually aligned (batched); = total_pairs in plain mode */
+  uint64_t n_edges;         /* sorted unique edges */
+  uint64_t ref_tiles;       /* reference 32x32 tiles over the aligned pairs */
+  double ms_candidates;     /* device time per stage, milliseconds (HIP events) */
+  double ms_align;
+  double ms_edges;
+  double ms_total;
+  uint32_t align_launches;  /* alignment kernel launches */
+  uint32_t host_syncs;      /* host<->device synchronisations inside the call */
+} ngia_graph_stats;
+
+typedef struct {
+  int32_t device;
+  int32_t cu_count;
+  int32_t clock_khz;
+  int32_t warp_size;
+  uint64_t hbm_total;
+  uint64_t hbm_free;
+  char name[128];
+  char arch[64];
+} ngia_device_info;
+
+/* ---------------------------------------------------------------- context */
+/* Replaces the device set-up in BuildGraph::check (buildGraph.hpp:100-116). */
+int ngia_open(int device, ngia_ctx** ctx);
+void ngia_close(ngia_ctx* ctx);
+/* Last error message of ctx (or of the calling thread when ctx is NULL). */
+const char* ngia_last_error(const ngia_ctx* ctx);
+int ngia_device_query(ngia_ctx* ctx, ngia_device_info* info);
+int ngia_sync(ngia_ctx* ctx);
+void ngia_free(void* p);
+
+/* ------------------------------------------- the three reference seams */
+/* (1a) minhash sketch: the packData inner loop (makedb.hpp:221-247) over all
+ * records.  packed: the packed area (records "L, ceil(L/32) x 4 planes");
+ * word_off[i]: uint32 word offset of record i inside it (buildGraph.hpp:173-177).
+ * sig: n x 64 uint32, sequence-major (the layout packData writes at signBase). */
+int ngia_sketch(ngia_ctx* ctx, const uint32_t* packed, uint64_t n_words,
+                const uint64_t* word_off, uint32_t n, uint32_t* sig);
+
+/* (1b) band bucketing: MakeDB::groupSignatures (makedb.hpp:289-364).
+ * sig as above; rep: 64 x n uint32, band-major, rep[b][i] = previous index with
+ * the same band-b signature, or i for a bucket head. */
+int ngia_bucket(ngia_ctx* ctx, const uint32_t* sig, uint32_t n, uint32_t* rep);
+
+/* (2) banded bit-parallel LCS: kernel_dynamic (buildGraph.hpp:192-253), same
+ * contract: jobs holds count triplets (rep, tar, score); score is written with
+ * lcs*100/(R+T-lcs) when lcs > (R+T-lcs)*threshold, else 0.  n_seqs bounds the
+ * rep/tar indices (checked). */
+int ngia_align(ngia_ctx* ctx, const uint32_t* packed, uint64_t n_words,
+               const uint64_t* word_off, uint32_t n_seqs, uint32_t* jobs,
+               uint32_t count, float threshold);
+
+/* (3) edge emission: BuildGraph::buildGraph (buildGraph.hpp:256-363).
+ * rep_rows: the first `bands` band-major rows; identity 1..99; bands 1..64.
+ * *edges receives the sorted unique edge list (release with ngia_free). */
+int ngia_build_graph(ngia_ctx* ctx, const uint32_t* packed, uint64_t n_words,
+                     const uint64_t* word_off, const uint32_t* lengths, uint32_t n,
+                     const uint32_t* rep_rows, uint32_t bands, uint32_t identity,
+                     uint32_t flags, ngia_edge** edges, uint64_t* n_edges,
+                     ngia_graph_stats* stats);
+
+/* ------------------------------------------- device-resident database */
+/* Upload the packed area, word offsets and lengths once (BuildGraph::readData,
+ * buildGraph.hpp:125-183, with explicit device buffers instead of managed
+ * memory).  Everything below works on the resident copy. */
+int ngia_db_create(ngia_ctx* ctx, const uint32_t* packed, uint64_t n_words,
+                   const uint64_t* word_off, const uint32_t* lengths, uint32_t n,
+                   ngia_db** db);
+void ngia_db_destroy(ngia_db* db);
+/* Sketch + bucket on the device (the makeDB half of the path). */
+int ngia_db_sketch(ngia_ctx* ctx, ngia_db* db);
+int ngia_db_bucket(ngia_ctx* ctx, ngia_db* db);
+/* Install representative rows read from a packed DB file instead. */
+int ngia_db_set_reps(ngia_ctx* ctx, ngia_db* db, const uint32_t* rep_rows, uint32_t bands);
+int ngia_db_get_sigs(ngia_ctx* ctx, ngia_db* db, uint32_t* sig);   /* n x 64 */
+int ngia_db_get_reps(ngia_ctx* ctx, ngia_db* db, uint32_t* rep);   /* 64 x n */
+/* Build the edge list on the device; it stays resident until the next call. */
+int ngia_db_graph(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
+                  uint32_t flags, ngia_graph_stats* stats);
+int ngia_db_get_edges(ngia_ctx* ctx, ngia_db* db, ngia_edge* out, uint64_t cap,
+                      uint64_t* n_edges);
+/* Scores of the distinct pairs of the last batched graph, sorted by (rep,tar):
+ * keys[i] = rep<<32 | tar.  Either array may be NULL. */
+int ngia_db_get_pair_scores(ngia_ctx* ctx, ngia_db* db, uint64_t* keys, uint32_t* scores,
+                            uint64_t cap, uint64_t* n_pairs);
+
+/* ------------------------------------------- fast mode (no graph, no Leiden) */
+/* FastClustering::cluster (fastClustering.hpp:249-352): per band, align the
+ * still-unclustered targets, parent[tar] = rep on a pass; cluster[i] is the
+ * representative index of i. */
+int ngia_db_fast(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
+                 uint32_t* cluster, uint64_t* total_pairs);
+
+/* ------------------------------------------- host side (no device work) */
+/* Leiden over the edge list (LeidenOmp::cluster, leidenParallel.hpp:265-322). */
+int ngia_leiden(const ngia_edge* edges, uint64_t m, int32_t n, float resolution,
+                uint32_t seed, int32_t iters, uint32_t* result);
+/* Leiden resolution from identity (main.cu:114-116). */
+float ngia_resolution(uint32_t identity);
+/* Result file (SaveResult::save, saveResult.hpp:31-83). */
+int ngia_save_result(const uint32_t* cluster, uint32_t n, const char* packed_path,
+                     const char* result_path);
+/* FASTA checks of MakeDB::check (makedb.hpp:59-91): the file exists, its first
+ * line has no trailing '\r', a non-empty file ends with '\n'.  NGIA_OK or
+ * NGIA_ERR_ARG with the reason in ngia_last_error(NULL).  No device work. */
+int ngia_fasta_check(const char* fasta_path);
+/* makeDB end to end: FASTA -> packed DB, sketch and bucket on the device
+ * (main.cu:54-83 -> makedb.hpp:59-364).  0 ok; NGIA_ERR_ARG when the FASTA
+ * fails the reference's checks (missing, CRLF first line, no final newline). */
+int ngia_makedb(ngia_ctx* ctx, const char* fasta_path, const char* packed_path);
+/* cluster end to end: packed DB -> edges -> Leiden -> result file
+ * (main.cu:86-127).  flags as for ngia_db_graph. */
+int ngia_cluster(ngia_ctx* ctx, const char* packed_path, const char* result_path,
+                 uint32_t identity, uint32_t bands, uint32_t flags, ngia_graph_stats* stats);
+
+/* Packed DB reader (Appendix layout of makedb.hpp:1-25): arrays are malloc'd,
+ * release with ngia_free.  word_off is relative to the packed area. */
+typedef struct {
+  uint32_t n;
+  uint32_t* name_len;
+  uint32_t* lengths;
+  uint64_t* pack_off;    /* absolute byte offsets as stored */
+  uint64_t* text_off;
+  uint32_t* rep_rows;    /* 64 x n */
+  uint32_t* packed;      /* the packed area */
+  uint64_t n_words;
+  uint64_t* word_off;    /* uint32 word offsets inside packed */
+} ngia_packed_db;
+int ngia_read_db(const char* path, uint32_t bands, ngia_packed_db* out);
+void ngia_release_db(ngia_packed_db* db);
+
+/* ------------------------------------------- synthetic families */
+/* No reference counterpart: seeded generator of the benchmark inputs. */
+typedef struct {
+  uint32_t n_families;
+  uint32_t members;
+  uint32_t dna;           /* 0: 20 amino acids, 1: ACGT */
+  uint32_t len_normal;    /* 0: log-normal(len_a = median, len_b = sigma); 1: normal(mean, sd) */
+  double len_a, len_b;
+  uint32_t len_min, len_max;
+  double tail_frac;       /* fraction of families drawn from the Pareto tail */
+  double tail_xmin, tail_alpha;
+  uint32_t tail_max;
+  double sub_lo, sub_hi;  /* per-member substitution rate ~ U[lo, hi] */
+  double indel_lo, indel_hi;
+  uint64_t seed;
+} ngia_gen_params;
+/* config 1..5 of the benchmark suite; scale multiplies the family count. */
+int ngia_gen_config(int config, double scale, uint64_t seed, ngia_gen_params* out);
+/* Write the families as FASTA (shuffled, one sequence line per record). */
+int64_t ngia_gen_fasta(const ngia_gen_params* p, const char* path);
+/* Same sequences, packed in memory in index order (length-descending stable
+ * sort of the FASTA order, exactly as makeDB would order them). */
+int ngia_gen_packed(const ngia_gen_params* p, ngia_packed_db* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

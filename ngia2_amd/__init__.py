@@ -1,0 +1,297 @@
+"""MI355X-native nGIA2 clustering hot path -- Python plumbing over the C-ABI.
+
+The product is the C-ABI library ``lib/libngia2.so`` (include/ngia2.h) and the
+``bin/ngia2`` command line (with ``makeDB`` / ``cluster`` aliases).  This
+module wraps the library with numpy arrays for tests and benchmarks; every
+call goes to the HIP implementation and raises ``NgiaError`` on failure.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (BANDS, GRAPH_BATCHED, GRAPH_PLAIN, NGIA_ERR_ARG, NGIA_ERR_IO,
+                   NGIA_ERR_NODEV, NGIA_ERR_STATE, NGIA_OK, BIN_DIR, LIB_PATH)
+
+__all__ = ["NgiaError", "Context", "Database", "PackedDB", "gen_config", "gen_fasta",
+           "gen_packed", "read_db", "leiden", "resolution", "save_result", "fasta_check",
+           "edges_to_array", "BANDS", "GRAPH_PLAIN", "GRAPH_BATCHED", "BIN_DIR", "LIB_PATH"]
+
+
+class NgiaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (status %d)" % (msg, code))
+        self.code = code
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _u64(a):
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def _check(rc, ctx=None):
+    if rc != NGIA_OK:
+        L = _lib.lib()
+        msg = L.ngia_last_error(ctx) if ctx else None
+        if not msg:
+            msg = L.ngia_last_error(None)
+        raise NgiaError(rc, (msg or b"").decode(errors="replace"))
+
+
+EDGE_DTYPE = np.dtype([("u", np.int32), ("v", np.int32), ("w", np.float32)])
+
+
+def edges_to_array(ptr, m):
+    if m == 0:
+        return np.zeros(0, dtype=EDGE_DTYPE)
+    buf = C.string_at(ptr, m * 12)
+    return np.frombuffer(buf, dtype=EDGE_DTYPE).copy()
+
+
+class Context:
+    """One HIP device and stream (ngia_open / ngia_close)."""
+
+    def __init__(self, device=0):
+        L = _lib.lib()
+        h = C.c_void_p()
+        _check(L.ngia_open(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _lib.lib().ngia_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        d = _lib.DeviceInfo()
+        _check(_lib.lib().ngia_device_query(self.h, C.byref(d)), self.h)
+        return {"device": d.device, "cu_count": d.cu_count, "clock_khz": d.clock_khz,
+                "warp_size": d.warp_size, "hbm_total": d.hbm_total, "hbm_free": d.hbm_free,
+                "name": d.name.decode(), "arch": d.arch.decode()}
+
+    # ---- the three reference seams, host pointers
+    def sketch(self, packed, word_off):
+        packed, word_off = _u32(packed), _u64(word_off)
+        n = len(word_off)
+        sig = np.zeros((n, BANDS), dtype=np.uint32)
+        _check(_lib.lib().ngia_sketch(self.h, _ptr(packed), len(packed), _ptr(word_off), n,
+                                      _ptr(sig)), self.h)
+        return sig
+
+    def bucket(self, sig):
+        sig = _u32(sig)
+        n = sig.shape[0]
+        rep = np.zeros((BANDS, n), dtype=np.uint32)
+        _check(_lib.lib().ngia_bucket(self.h, _ptr(sig), n, _ptr(rep)), self.h)
+        return rep
+
+    def align(self, packed, word_off, jobs, threshold):
+        """kernel_dynamic contract: jobs (count,3) uint32, column 2 overwritten."""
+        packed, word_off = _u32(packed), _u64(word_off)
+        jobs = np.array(jobs, dtype=np.uint32, order="C", copy=True)
+        _check(_lib.lib().ngia_align(self.h, _ptr(packed), len(packed), _ptr(word_off),
+                                     len(word_off), _ptr(jobs), len(jobs),
+                                     C.c_float(threshold)), self.h)
+        return jobs
+
+    def build_graph(self, packed, word_off, lengths, rep_rows, bands, identity,
+                    mode=GRAPH_BATCHED):
+        packed, word_off, lengths = _u32(packed), _u64(word_off), _u32(lengths)
+        rep_rows = _u32(rep_rows)
+        ep = C.POINTER(_lib.Edge)()
+        m = C.c_uint64()
+        st = _lib.GraphStats()
+        _check(_lib.lib().ngia_build_graph(self.h, _ptr(packed), len(packed), _ptr(word_off),
+                                           _ptr(lengths), len(lengths), _ptr(rep_rows), bands,
+                                           identity, mode, C.byref(ep), C.byref(m),
+                                           C.byref(st)), self.h)
+        try:
+            arr = edges_to_array(ep, m.value)
+        finally:
+            _lib.lib().ngia_free(ep)
+        return arr, st.as_dict()
+
+    def makedb(self, fasta, packed):
+        _check(_lib.lib().ngia_makedb(self.h, fasta.encode(), packed.encode()), self.h)
+
+    def cluster(self, packed, result, identity, bands=BANDS, mode=GRAPH_BATCHED):
+        st = _lib.GraphStats()
+        _check(_lib.lib().ngia_cluster(self.h, packed.encode(), result.encode(), identity, bands,
+                                       mode, C.byref(st)), self.h)
+        return st.as_dict()
+
+
+class Database:
+    """A device-resident packed database (ngia_db_*)."""
+
+    def __init__(self, ctx, packed, word_off, lengths=None):
+        self.ctx = ctx
+        self._packed, self._word_off = _u32(packed), _u64(word_off)
+        self._lengths = None if lengths is None else _u32(lengths)
+        self.n = len(self._word_off)
+        h = C.c_void_p()
+        _check(_lib.lib().ngia_db_create(ctx.h, _ptr(self._packed), len(self._packed),
+                                         _ptr(self._word_off), _ptr(self._lengths), self.n,
+                                         C.byref(h)), ctx.h)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _lib.lib().ngia_db_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sketch(self):
+        _check(_lib.lib().ngia_db_sketch(self.ctx.h, self.h), self.ctx.h)
+
+    def bucket(self):
+        _check(_lib.lib().ngia_db_bucket(self.ctx.h, self.h), self.ctx.h)
+
+    def set_reps(self, rep_rows, bands):
+        rep_rows = _u32(rep_rows)
+        _check(_lib.lib().ngia_db_set_reps(self.ctx.h, self.h, _ptr(rep_rows), bands), self.ctx.h)
+
+    def sigs(self):
+        s = np.zeros((self.n, BANDS), dtype=np.uint32)
+        _check(_lib.lib().ngia_db_get_sigs(self.ctx.h, self.h, _ptr(s)), self.ctx.h)
+        return s
+
+    def reps(self):
+        r = np.zeros((BANDS, self.n), dtype=np.uint32)
+        _check(_lib.lib().ngia_db_get_reps(self.ctx.h, self.h, _ptr(r)), self.ctx.h)
+        return r
+
+    def graph(self, identity, bands=BANDS, mode=GRAPH_BATCHED):
+        st = _lib.GraphStats()
+        _check(_lib.lib().ngia_db_graph(self.ctx.h, self.h, identity, bands, mode, C.byref(st)),
+               self.ctx.h)
+        return st.as_dict()
+
+    def edges(self):
+        m = C.c_uint64()
+        L = _lib.lib()
+        _check(L.ngia_db_get_edges(self.ctx.h, self.h, None, 0, C.byref(m)), self.ctx.h)
+        out = np.zeros(m.value, dtype=EDGE_DTYPE)
+        if m.value:
+            _check(L.ngia_db_get_edges(self.ctx.h, self.h, _ptr(out), m.value, C.byref(m)),
+                   self.ctx.h)
+        return out
+
+    def pair_scores(self):
+        m = C.c_uint64()
+        L = _lib.lib()
+        _check(L.ngia_db_get_pair_scores(self.ctx.h, self.h, None, None, 0, C.byref(m)),
+               self.ctx.h)
+        keys = np.zeros(m.value, dtype=np.uint64)
+        sc = np.zeros(m.value, dtype=np.uint32)
+        if m.value:
+            _check(L.ngia_db_get_pair_scores(self.ctx.h, self.h, _ptr(keys), _ptr(sc), m.value,
+                                             C.byref(m)), self.ctx.h)
+        return keys, sc
+
+    def fast(self, identity, bands=BANDS):
+        cl = np.zeros(self.n, dtype=np.uint32)
+        tot = C.c_uint64()
+        _check(_lib.lib().ngia_db_fast(self.ctx.h, self.h, identity, bands, _ptr(cl),
+                                       C.byref(tot)), self.ctx.h)
+        return cl, tot.value
+
+
+class PackedDB:
+    """A packed DB in host memory (ngia_packed_db), copied into numpy arrays."""
+
+    @classmethod
+    def _from_raw(cls, raw, bands):
+        db = cls()
+        n = raw.n
+
+        def arr(p, count):
+            if count == 0 or not p:
+                return None
+            return np.ctypeslib.as_array(p, shape=(count,)).copy()
+
+        db.n = n
+        db.name_len = arr(raw.name_len, n)
+        db.lengths = arr(raw.lengths, n)
+        db.pack_off = arr(raw.pack_off, n)
+        db.text_off = arr(raw.text_off, n)
+        db.word_off = arr(raw.word_off, n)
+        db.packed = arr(raw.packed, raw.n_words)
+        rep = arr(raw.rep_rows, bands * n) if bands else None
+        db.rep_rows = None if rep is None else rep.reshape(bands, n)
+        _lib.lib().ngia_release_db(C.byref(raw))
+        return db
+
+
+def read_db(path, bands=BANDS):
+    raw = _lib.PackedDB()
+    _check(_lib.lib().ngia_read_db(path.encode(), bands, C.byref(raw)))
+    return PackedDB._from_raw(raw, bands)
+
+
+def gen_config(config, scale=1.0, seed=1):
+    p = _lib.GenParams()
+    _check(_lib.lib().ngia_gen_config(config, scale, seed, C.byref(p)))
+    return p
+
+
+def gen_fasta(params, path):
+    n = _lib.lib().ngia_gen_fasta(C.byref(params), path.encode())
+    if n < 0:
+        _check(int(n))
+    return int(n)
+
+
+def gen_packed(params):
+    raw = _lib.PackedDB()
+    _check(_lib.lib().ngia_gen_packed(C.byref(params), C.byref(raw)))
+    return PackedDB._from_raw(raw, 0)
+
+
+def leiden(edges, n, resolution_, seed=42, iters=1):
+    edges = np.ascontiguousarray(edges, dtype=EDGE_DTYPE)
+    out = np.zeros(n, dtype=np.uint32)
+    _check(_lib.lib().ngia_leiden(_ptr(edges), len(edges), n, C.c_float(resolution_), seed, iters,
+                                  _ptr(out)))
+    return out
+
+
+def resolution(identity):
+    return _lib.lib().ngia_resolution(identity)
+
+
+def save_result(cluster, packed_path, result_path):
+    cluster = _u32(cluster)
+    _check(_lib.lib().ngia_save_result(_ptr(cluster), len(cluster), packed_path.encode(),
+                                       result_path.encode()))
+
+
+def fasta_check(path):
+    rc = _lib.lib().ngia_fasta_check(path.encode())
+    if rc != NGIA_OK:
+        return _lib.lib().ngia_last_error(None).decode()
+    return None

@@ -1,0 +1,749 @@
+// capi.hip -- the C-ABI of include/ngia2.h: context, device-resident
+// database, and the plain / batched graph pipelines.
+//
+// Plain form (buildGraph.hpp:274-350): per band, candidates on the device,
+// the job count copied to the host, one alignment launch, a host sync, the
+// passing jobs appended to the raw edge list; then one sort+unique.
+// Batched form: every band's candidates at once, deduplicated per target,
+// binned by band-window width, aligned by the length-specialised persistent
+// kernels, passing pairs appended to the raw edge list -- all captured once
+// in a hipGraph and replayed; then the same sort+unique.
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "dev_common.hpp"
+#include "dev_graph.hpp"
+#include "host_common.hpp"
+
+using namespace ngia;
+
+namespace ngia {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // Grow-only; contents are not preserved.
+  void ensure(size_t b) {
+    if (b <= bytes && p) return;
+    release();
+    size_t want = b < 256 ? 256 : (b + 255) & ~(size_t)255;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw HipError{NGIA_ERR_NOMEM, "hipMalloc(" + std::to_string(want) + " bytes) failed: " +
+                                         hipGetErrorString(e)};
+    }
+    bytes = want;
+  }
+  template <class T>
+  T* as() const { return (T*)p; }
+};
+
+// Device counters of one graph round, cleared by a fill kernel.
+struct Counters {
+  unsigned long long total_cand;  // candidates with duplicates
+  unsigned long long ecount;      // raw edges
+  unsigned long long tiles;       // reference tiles of the aligned pairs
+  unsigned long long unique;      // edges after unique
+  uint32_t bin_cnt[kBins];
+  uint32_t bin_off[kBins + 1];
+  uint32_t cursor[kBins];
+  uint32_t work[kBins];
+};
+
+}  // namespace ngia
+
+struct ngia_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t prop{};
+  std::string err;
+  hipEvent_t ev[8] = {};
+  uint32_t wave_kmax = 8;  // largest k_align_wave<K> usable on this device
+  ~ngia_ctx() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct ngia_db {
+  ngia_ctx* ctx = nullptr;
+  uint32_t n = 0;
+  uint64_t n_words = 0;
+  uint32_t max_len = 0;
+  uint32_t rep_bands = 0;
+  bool have_sig = false;
+  DevBuf packed, word_off, lengths, codes, code_off, sig, rep;
+  // graph workspace
+  DevBuf flags, pos, sums, jobs, ekeys, escores, ekeys_tmp, escores_tmp, fin_temp;
+  DevBuf counters, cnt, off, pairs, score, bin_of, list, plain_scratch, plain_jobs, bucket_scratch;
+  uint64_t n_edges = 0, n_pairs = 0;
+  // the captured batched round
+  hipGraphExec_t gexec = nullptr;
+  uint32_t g_identity = 0, g_bands = 0;
+  ~ngia_db() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+  }
+};
+
+namespace {
+
+int fail(ngia_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  set_thread_error(msg);
+  return code;
+}
+
+template <class F>
+int guarded(ngia_ctx* ctx, F&& f) {
+  try {
+    return f();
+  } catch (const HipError& e) {
+    return fail(ctx, e.code, e.msg);
+  } catch (const std::bad_alloc&) {
+    return fail(ctx, NGIA_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(ctx, NGIA_ERR_ARG, e.what());
+  }
+}
+
+void h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes) NGIA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+}
+void d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes) NGIA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+}
+
+// Validate the packed records before any kernel touches them: every record
+// lies inside the area, its length word matches `lengths` (when given) and is
+// below 0xFFFF (the reference's lines[2048] bound).
+void validate_records(const uint32_t* packed, uint64_t n_words, const uint64_t* word_off,
+                      const uint32_t* lengths, uint32_t n, uint32_t* max_len) {
+  uint32_t mx = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    if (word_off[i] >= n_words) throw HipError{NGIA_ERR_ARG, "record offset outside packed area"};
+    const uint32_t L = packed[word_off[i]];
+    if (L >= 0xFFFF) throw HipError{NGIA_ERR_ARG, "record length >= 65535"};
+    if (word_off[i] + rec_words(L) > n_words)
+      throw HipError{NGIA_ERR_ARG, "record extends past packed area"};
+    if (lengths && lengths[i] != L) throw HipError{NGIA_ERR_ARG, "lengths[] disagrees with record"};
+    mx = std::max(mx, L);
+  }
+  *max_len = mx;
+}
+
+float event_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  NGIA_HIP(hipEventElapsedTime(&ms, a, b));
+  return ms;
+}
+
+uint32_t persistent_blocks(const ngia_ctx* ctx, size_t lds_bytes) {
+  const uint32_t cu = (uint32_t)std::max(1, ctx->prop.multiProcessorCount);
+  uint32_t per_cu = lds_bytes ? (uint32_t)std::min<size_t>(16, (160 * 1024) / lds_bytes) : 16;
+  if (per_cu < 1) per_cu = 1;
+  return cu * per_cu;
+}
+
+void db_upload(ngia_ctx* ctx, ngia_db* db, const uint32_t* packed, uint64_t n_words,
+               const uint64_t* word_off, const uint32_t* lengths, uint32_t n) {
+  std::vector<uint32_t> len_v;
+  if (!lengths) {
+    len_v.resize(n);
+    for (uint32_t i = 0; i < n; i++) len_v[i] = packed[word_off[i]];
+    lengths = len_v.data();
+  }
+  validate_records(packed, n_words, word_off, lengths, n, &db->max_len);
+  db->n = n;
+  db->n_words = n_words;
+  hipStream_t s = ctx->stream;
+  db->packed.ensure(4 * std::max<uint64_t>(n_words, 1));
+  db->word_off.ensure(8ull * std::max<uint32_t>(n, 1));
+  db->lengths.ensure(4ull * std::max<uint32_t>(n, 1));
+  h2d(db->packed.p, packed, 4 * n_words, s);
+  h2d(db->word_off.p, word_off, 8ull * n, s);
+  h2d(db->lengths.p, lengths, 4ull * n, s);
+  // byte codes: 32 bytes per 32-residue block, sequence i at code_off[i]
+  std::vector<uint64_t> coff(std::max<uint32_t>(n, 1));
+  uint64_t acc = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    coff[i] = acc;
+    acc += 32ull * ((lengths[i] + 31) / 32);
+  }
+  db->code_off.ensure(8ull * std::max<uint32_t>(n, 1));
+  db->codes.ensure(acc + 64);
+  h2d(db->code_off.p, coff.data(), 8ull * n, s);
+  launch_codes(db->packed.as<uint32_t>(), db->word_off.as<uint64_t>(), db->code_off.as<uint64_t>(),
+               n, db->codes.as<uint8_t>(), s);
+  NGIA_HIP(hipStreamSynchronize(s));
+  db->have_sig = false;
+  db->rep_bands = 0;
+  if (db->gexec) {
+    (void)hipGraphExecDestroy(db->gexec);
+    db->gexec = nullptr;
+  }
+}
+
+AlignArgs align_args(const ngia_db* db, float thr) {
+  AlignArgs a;
+  a.packed = db->packed.as<uint32_t>();
+  a.word_off = db->word_off.as<uint64_t>();
+  a.codes = db->codes.as<uint8_t>();
+  a.code_off = db->code_off.as<uint64_t>();
+  a.thr = thr;
+  return a;
+}
+
+// Capacity of the raw edge list: every candidate of every band, duplicates included.
+void ensure_edge_space(ngia_db* db, uint64_t m) {
+  m = std::max<uint64_t>(m, 1);
+  db->ekeys.ensure(8 * m);
+  db->escores.ensure(4 * m);
+  db->ekeys_tmp.ensure(8 * m);
+  db->escores_tmp.ensure(4 * m);
+  db->fin_temp.ensure(finalize_scratch_bytes(m));
+}
+
+void finalize(ngia_ctx* ctx, ngia_db* db, uint64_t m) {
+  Counters* dc = db->counters.as<Counters>();
+  finalize_edges(db->ekeys.as<uint64_t>(), db->escores.as<uint32_t>(), m,
+                 db->ekeys_tmp.as<uint64_t>(), db->escores_tmp.as<uint32_t>(), db->fin_temp.p,
+                 db->fin_temp.bytes, &dc->unique, ctx->stream);
+  unsigned long long u = 0;
+  d2h(&u, &dc->unique, 8, ctx->stream);
+  NGIA_HIP(hipStreamSynchronize(ctx->stream));
+  db->n_edges = u;
+}
+
+void graph_plain(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
+                 ngia_graph_stats* st, const uint32_t* parent /* fast mode */) {
+  hipStream_t s = ctx->stream;
+  const uint32_t n = db->n;
+  const float thr = (float)identity / 100.0f;
+  db->flags.ensure(4ull * n);
+  db->pos.ensure(8ull * (n + 1));
+  db->sums.ensure(8 * scan_scratch_words(n));
+  db->jobs.ensure(12ull * n);
+  db->counters.ensure(sizeof(Counters));
+  const uint32_t nw_max = (db->max_len + 31) / 32;
+  const uint64_t sw = align_plain_scratch_words(nw_max);
+  db->plain_scratch.ensure(4 * sw);
+  if (!parent) ensure_edge_space(db, (uint64_t)n * bands);
+  fill_u32(db->counters.as<uint32_t>(), 0, sizeof(Counters) / 4, s);
+  Counters* dc = db->counters.as<Counters>();
+  const AlignArgs aa = align_args(db, thr);
+  uint64_t total = 0;
+  float ms_cand = 0, ms_align = 0, ms_edges = 0;
+  uint32_t launches = 0, syncs = 0;
+  for (uint32_t b = 0; b < bands; b++) {
+    const uint32_t* rep = db->rep.as<uint32_t>() + (uint64_t)b * n;
+    NGIA_HIP(hipEventRecord(ctx->ev[0], s));
+    band_jobs(rep, db->lengths.as<uint32_t>(), n, thr, parent, db->flags.as<uint32_t>(),
+              db->pos.as<uint64_t>(), db->sums.as<uint64_t>(), db->jobs.as<uint32_t>(), s);
+    uint64_t count = 0;
+    d2h(&count, db->pos.as<uint64_t>() + n, 8, s);
+    NGIA_HIP(hipEventRecord(ctx->ev[1], s));
+    NGIA_HIP(hipStreamSynchronize(s));  // the host needs the count to size the launch
+    syncs++;
+    ms_cand += event_ms(ctx->ev[0], ctx->ev[1]);
+    if (count == 0) continue;
+    total += count;
+    launch_align_plain(aa, db->jobs.as<uint32_t>(), (uint32_t)count, nw_max,
+                       db->plain_scratch.as<uint32_t>(), sw, s);
+    launches++;
+    NGIA_HIP(hipEventRecord(ctx->ev[2], s));
+    NGIA_HIP(hipStreamSynchronize(s));  // buildGraph.hpp:333
+    syncs++;
+    if (parent) {
+      fast_apply(db->jobs.as<uint32_t>(), (uint32_t)count, const_cast<uint32_t*>(parent), s);
+    } else {
+      jobs_to_edges(db->jobs.as<uint32_t>(), (uint32_t)count, db->ekeys.as<uint64_t>(),
+                    db->escores.as<uint32_t>(), &dc->ecount, s);
+    }
+    NGIA_HIP(hipEventRecord(ctx->ev[3], s));
+    ms_align += event_ms(ctx->ev[1], ctx->ev[2]);
+    NGIA_HIP(hipEventSynchronize(ctx->ev[3]));
+    ms_edges += event_ms(ctx->ev[2], ctx->ev[3]);
+  }
+  if (st) {
+    st->total_pairs = total;
+    st->distinct_pairs = total;
+    st->ms_candidates = ms_cand;
+    st->ms_align = ms_align;
+    st->align_launches = launches;
+    st->host_syncs = syncs;
+  }
+  if (parent) return;
+  unsigned long long ecount = 0;
+  d2h(&ecount, &dc->ecount, 8, s);
+  NGIA_HIP(hipEventRecord(ctx->ev[4], s));
+  NGIA_HIP(hipStreamSynchronize(s));
+  finalize(ctx, db, ecount);
+  NGIA_HIP(hipEventRecord(ctx->ev[5], s));
+  NGIA_HIP(hipEventSynchronize(ctx->ev[5]));
+  ms_edges += event_ms(ctx->ev[4], ctx->ev[5]);
+  if (st) {
+    st->ms_edges = ms_edges;
+    st->n_edges = db->n_edges;
+  }
+}
+
+// Enqueue one batched round (capturable: fixed grids, device-side counts).
+void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands) {
+  hipStream_t s = ctx->stream;
+  const uint32_t n = db->n;
+  const float thr = (float)identity / 100.0f;
+  Counters* dc = db->counters.as<Counters>();
+  const uint32_t* len = db->lengths.as<uint32_t>();
+  fill_u32(db->counters.as<uint32_t>(), 0, sizeof(Counters) / 4, s);
+  pair_count(db->rep.as<uint32_t>(), len, n, bands, thr, db->cnt.as<uint32_t>(), &dc->total_cand, s);
+  scan_u32(db->cnt.as<uint32_t>(), n, db->off.as<uint64_t>(), db->sums.as<uint64_t>(), s);
+  pair_emit(db->rep.as<uint32_t>(), len, n, bands, thr, db->off.as<uint64_t>(),
+            db->pairs.as<uint2>(), s);
+  const uint64_t* npairs = db->off.as<uint64_t>() + n;
+  const uint32_t grid = persistent_blocks(ctx, 0) * 4;
+  bin_pairs(db->pairs.as<uint2>(), npairs, len, thr, db->bin_of.as<uint32_t>(), dc->bin_cnt,
+            dc->bin_off, dc->cursor, db->list.as<uint32_t>(), &dc->tiles, db->score.as<uint32_t>(),
+            ctx->wave_kmax, grid, s);
+  const AlignArgs aa = align_args(db, thr);
+  // widest windows first: their pairs take longest
+  for (int k = 5; k >= 0; k--) {
+    if ((1u << k) > ctx->wave_kmax) continue;
+    const int bin = kBinWave0 + k;
+    launch_align_wave(1 << k, aa, db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[bin],
+                      &dc->bin_cnt[bin], &dc->work[bin], db->score.as<uint32_t>(),
+                      persistent_blocks(ctx, wave_lds_bytes(1 << k)), s);
+  }
+  for (int W = kWinMax; W >= 2; W--) {
+    launch_align_win(W, aa, db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[W],
+                     &dc->bin_cnt[W], &dc->work[W], db->score.as<uint32_t>(),
+                     persistent_blocks(ctx, win_lds_bytes(W)), s);
+  }
+  // windows beyond 64*32 words: the plain kernel over job triplets
+  plain_bin_jobs(db->pairs.as<uint2>(), db->list.as<uint32_t>(), dc->bin_off, dc->bin_cnt,
+                 db->plain_jobs.as<uint32_t>(), s);
+  const uint32_t nw_max = (db->max_len + 31) / 32;
+  launch_align_plain(aa, db->plain_jobs.as<uint32_t>(), 0xFFFFFFFFu, nw_max,
+                     db->plain_scratch.as<uint32_t>(), db->plain_scratch.bytes / 4, s,
+                     &dc->bin_cnt[kBinPlain]);
+  plain_bin_scores(db->list.as<uint32_t>(), dc->bin_off, dc->bin_cnt, db->plain_jobs.as<uint32_t>(),
+                   db->score.as<uint32_t>(), s);
+  pairs_to_edges(db->pairs.as<uint2>(), npairs, db->score.as<uint32_t>(), db->ekeys.as<uint64_t>(),
+                 db->escores.as<uint32_t>(), &dc->ecount, grid, s);
+}
+
+void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
+                   ngia_graph_stats* st) {
+  hipStream_t s = ctx->stream;
+  const uint32_t n = db->n;
+  const uint64_t cap = (uint64_t)n * bands;  // distinct pairs <= candidates <= n*bands
+  db->counters.ensure(sizeof(Counters));
+  db->cnt.ensure(4ull * n);
+  db->off.ensure(8ull * (n + 1));
+  db->sums.ensure(8 * scan_scratch_words(n));
+  db->pairs.ensure(8 * std::max<uint64_t>(cap, 1));
+  db->score.ensure(4 * std::max<uint64_t>(cap, 1));
+  db->bin_of.ensure(4 * std::max<uint64_t>(cap, 1));
+  db->list.ensure(4 * std::max<uint64_t>(cap, 1));
+  const uint32_t nw_max = (db->max_len + 31) / 32;
+  // The plain bin only holds windows wider than 2048 words; its job list and
+  // row scratch are sized for the whole pair list so the graph never grows.
+  db->plain_jobs.ensure(12 * std::max<uint64_t>(cap, 1));
+  db->plain_scratch.ensure(4 * align_plain_scratch_words(nw_max));
+  ensure_edge_space(db, cap);
+  const bool reuse = db->gexec && db->g_identity == identity && db->g_bands == bands;
+  NGIA_HIP(hipEventRecord(ctx->ev[0], s));
+  if (!reuse) {
+    if (db->gexec) {
+      (void)hipGraphExecDestroy(db->gexec);
+      db->gexec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    NGIA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_batched(ctx, db, identity, bands);
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    NGIA_HIP(hipStreamEndCapture(s, &g));
+    hipError_t e = hipGraphInstantiate(&db->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    NGIA_HIP(e);
+    db->g_identity = identity;
+    db->g_bands = bands;
+  }
+  NGIA_HIP(hipGraphLaunch(db->gexec, s));
+  NGIA_HIP(hipEventRecord(ctx->ev[1], s));
+  Counters hc;
+  d2h(&hc, db->counters.p, sizeof(Counters), s);
+  uint64_t npairs = 0;
+  d2h(&npairs, db->off.as<uint64_t>() + n, 8, s);
+  NGIA_HIP(hipStreamSynchronize(s));
+  db->n_pairs = npairs;
+  NGIA_HIP(hipEventRecord(ctx->ev[4], s));
+  finalize(ctx, db, hc.ecount);
+  NGIA_HIP(hipEventRecord(ctx->ev[5], s));
+  NGIA_HIP(hipEventSynchronize(ctx->ev[5]));
+  if (st) {
+    st->total_pairs = hc.total_cand;
+    st->distinct_pairs = npairs;
+    st->n_edges = db->n_edges;
+    st->ref_tiles = hc.tiles;
+    st->ms_candidates = 0;
+    st->ms_align = event_ms(ctx->ev[0], ctx->ev[1]);
+    st->ms_edges = event_ms(ctx->ev[4], ctx->ev[5]);
+    st->align_launches = 6 + (kWinMax - 1) + 1;
+    st->host_syncs = 2;
+  }
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+extern "C" int ngia_open(int device, ngia_ctx** out) {
+  if (!out) return fail(nullptr, NGIA_ERR_ARG, "null ctx pointer");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return fail(nullptr, NGIA_ERR_NODEV, "no HIP device");
+  if (device < 0 || device >= count) return fail(nullptr, NGIA_ERR_ARG, "device out of range");
+  auto ctx = std::make_unique<ngia_ctx>();
+  ctx->device = device;
+  int rc = guarded(ctx.get(), [&] {
+    NGIA_HIP(hipSetDevice(device));
+    NGIA_HIP(hipGetDeviceProperties(&ctx->prop, device));
+    ctx->wave_kmax = (uint32_t)align_kernels_setup(ctx->prop.sharedMemPerBlock);
+    NGIA_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    for (auto& e : ctx->ev) NGIA_HIP(hipEventCreate(&e));
+    return NGIA_OK;
+  });
+  if (rc != NGIA_OK) return rc;
+  *out = ctx.release();
+  return NGIA_OK;
+}
+
+extern "C" void ngia_close(ngia_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+}
+
+extern "C" const char* ngia_last_error(const ngia_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : thread_error();
+}
+
+extern "C" int ngia_device_query(ngia_ctx* ctx, ngia_device_info* info) {
+  if (!ctx || !info) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  return guarded(ctx, [&] {
+    std::memset(info, 0, sizeof(*info));
+    info->device = ctx->device;
+    info->cu_count = ctx->prop.multiProcessorCount;
+    info->clock_khz = ctx->prop.clockRate;
+    info->warp_size = ctx->prop.warpSize;
+    size_t fr = 0, tot = 0;
+    NGIA_HIP(hipMemGetInfo(&fr, &tot));
+    info->hbm_free = fr;
+    info->hbm_total = tot;
+    std::snprintf(info->name, sizeof(info->name), "%s", ctx->prop.name);
+    std::snprintf(info->arch, sizeof(info->arch), "%s", ctx->prop.gcnArchName);
+    return NGIA_OK;
+  });
+}
+
+extern "C" int ngia_sync(ngia_ctx* ctx) {
+  if (!ctx) return fail(ctx, NGIA_ERR_ARG, "null ctx");
+  return guarded(ctx, [&] {
+    NGIA_HIP(hipStreamSynchronize(ctx->stream));
+    return NGIA_OK;
+  });
+}
+
+extern "C" int ngia_db_create(ngia_ctx* ctx, const uint32_t* packed, uint64_t n_words,
+                              const uint64_t* word_off, const uint32_t* lengths, uint32_t n,
+                              ngia_db** out) {
+  if (!ctx || !out || (n && (!packed || !word_off))) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  if (n >= 0x7FFFFFFFu) return fail(ctx, NGIA_ERR_ARG, "too many sequences");
+  *out = nullptr;
+  auto db = std::make_unique<ngia_db>();
+  db->ctx = ctx;
+  int rc = guarded(ctx, [&] {
+    db_upload(ctx, db.get(), packed, n_words, word_off, lengths, n);
+    return NGIA_OK;
+  });
+  if (rc != NGIA_OK) return rc;
+  *out = db.release();
+  return NGIA_OK;
+}
+
+extern "C" void ngia_db_destroy(ngia_db* db) {
+  if (!db) return;
+  if (db->ctx) (void)hipStreamSynchronize(db->ctx->stream);
+  delete db;
+}
+
+extern "C" int ngia_db_sketch(ngia_ctx* ctx, ngia_db* db) {
+  if (!ctx || !db) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  return guarded(ctx, [&] {
+    db->sig.ensure(256ull * std::max<uint32_t>(db->n, 1));
+    launch_sketch(db->codes.as<uint8_t>(), db->code_off.as<uint64_t>(), db->lengths.as<uint32_t>(),
+                  db->n, db->sig.as<uint32_t>(), ctx->stream);
+    NGIA_HIP(hipStreamSynchronize(ctx->stream));
+    db->have_sig = true;
+    return NGIA_OK;
+  });
+}
+
+extern "C" int ngia_db_bucket(ngia_ctx* ctx, ngia_db* db) {
+  if (!ctx || !db) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  if (!db->have_sig) return fail(ctx, NGIA_ERR_STATE, "bucket before sketch");
+  return guarded(ctx, [&] {
+    db->rep.ensure(256ull * std::max<uint32_t>(db->n, 1));
+    db->bucket_scratch.ensure(bucket_scratch_bytes(db->n));
+    launch_bucket(db->sig.as<uint32_t>(), db->n, db->rep.as<uint32_t>(), db->bucket_scratch.p,
+                  ctx->stream);
+    NGIA_HIP(hipStreamSynchronize(ctx->stream));
+    db->rep_bands = NGIA_BANDS;
+    if (db->gexec) {
+      (void)hipGraphExecDestroy(db->gexec);
+      db->gexec = nullptr;
+    }
+    return NGIA_OK;
+  });
+}
+
+extern "C" int ngia_db_set_reps(ngia_ctx* ctx, ngia_db* db, const uint32_t* rep_rows,
+                                uint32_t bands) {
+  if (!ctx || !db || !rep_rows) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  if (bands < 1 || bands > NGIA_BANDS) return fail(ctx, NGIA_ERR_ARG, "bands should be 1-64");
+  const uint64_t m = (uint64_t)bands * db->n;
+  for (uint64_t k = 0; k < m; k++)
+    if (rep_rows[k] >= db->n) return fail(ctx, NGIA_ERR_ARG, "representative index out of range");
+  return guarded(ctx, [&] {
+    db->rep.ensure(256ull * std::max<uint32_t>(db->n, 1));
+    h2d(db->rep.p, rep_rows, 4 * m, ctx->stream);
+    NGIA_HIP(hipStreamSynchronize(ctx->stream));
+    db->rep_bands = bands;
+    if (db->gexec) {
+      (void)hipGraphExecDestroy(db->gexec);
+      db->gexec = nullptr;
+    }
+    return NGIA_OK;
+  });
+}
+
+extern "C" int ngia_db_get_sigs(ngia_ctx* ctx, ngia_db* db, uint32_t* sig) {
+  if (!ctx || !db || !sig) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  if (!db->have_sig) return fail(ctx, NGIA_ERR_STATE, "no signatures");
+  return guarded(ctx, [&] {
+    d2h(sig, db->sig.p, 256ull * db->n, ctx->stream);
+    NGIA_HIP(hipStreamSynchronize(ctx->stream));
+    return NGIA_OK;
+  });
+}
+
+extern "C" int ngia_db_get_reps(ngia_ctx* ctx, ngia_db* db, uint32_t* rep) {
+  if (!ctx || !db || !rep) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  if (!db->rep_bands) return fail(ctx, NGIA_ERR_STATE, "no representative rows");
+  return guarded(ctx, [&] {
+    d2h(rep, db->rep.p, 4ull * db->rep_bands * db->n, ctx->stream);
+    NGIA_HIP(hipStreamSynchronize(ctx->stream));
+    return NGIA_OK;
+  });
+}
+
+extern "C" int ngia_db_graph(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
+                             uint32_t flags, ngia_graph_stats* st) {
+  if (!ctx || !db) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  if (identity < 1 || identity > 99) return fail(ctx, NGIA_ERR_ARG, "identity should be 1-99");
+  if (bands < 1 || bands > NGIA_BANDS) return fail(ctx, NGIA_ERR_ARG, "bands should be 1-64");
+  if (bands > db->rep_bands) return fail(ctx, NGIA_ERR_STATE, "fewer representative rows than bands");
+  if (db->n == 0) return fail(ctx, NGIA_ERR_ARG, "empty database");
+  if (st) std::memset(st, 0, sizeof(*st));
+  return guarded(ctx, [&] {
+    auto t0 = std::chrono::steady_clock::now();
+    if (flags == NGIA_GRAPH_BATCHED)
+      graph_batched(ctx, db, identity, bands, st);
+    else
+      graph_plain(ctx, db, identity, bands, st, nullptr);
+    auto t1 = std::chrono::steady_clock::now();
+    if (st) st->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return NGIA_OK;
+  });
+}
+
+extern "C" int ngia_db_get_edges(ngia_ctx* ctx, ngia_db* db, ngia_edge* out, uint64_t cap,
+                                 uint64_t* n_edges) {
+  if (!ctx || !db || !n_edges) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  *n_edges = db->n_edges;
+  if (!out) return NGIA_OK;
+  if (cap < db->n_edges) return fail(ctx, NGIA_ERR_ARG, "edge buffer too small");
+  return guarded(ctx, [&] {
+    const uint64_t m = db->n_edges;
+    std::vector<uint64_t> k(m);
+    std::vector<uint32_t> sc(m);
+    d2h(k.data(), db->ekeys.p, 8 * m, ctx->stream);
+    d2h(sc.data(), db->escores.p, 4 * m, ctx->stream);
+    NGIA_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint64_t e = 0; e < m; e++) {
+      out[e].u = (int32_t)(uint32_t)(k[e] >> 32);
+      out[e].v = (int32_t)(uint32_t)k[e];
+      out[e].w = (float)sc[e] / 100.0f;  // buildGraph.hpp:346
+    }
+    return NGIA_OK;
+  });
+}
+
+extern "C" int ngia_db_get_pair_scores(ngia_ctx* ctx, ngia_db* db, uint64_t* keys,
+                                       uint32_t* scores, uint64_t cap, uint64_t* n_pairs) {
+  if (!ctx || !db || !n_pairs) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  *n_pairs = db->n_pairs;
+  if (!keys && !scores) return NGIA_OK;
+  if (cap < db->n_pairs) return fail(ctx, NGIA_ERR_ARG, "pair buffer too small");
+  return guarded(ctx, [&] {
+    const uint64_t m = db->n_pairs;
+    std::vector<uint2> pr(m);
+    std::vector<uint32_t> sc(m);
+    d2h(pr.data(), db->pairs.p, 8 * m, ctx->stream);
+    d2h(sc.data(), db->score.p, 4 * m, ctx->stream);
+    NGIA_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<uint64_t> order(m);
+    for (uint64_t i = 0; i < m; i++) order[i] = ((uint64_t)pr[i].x << 32) | pr[i].y;
+    std::vector<uint64_t> idx(m);
+    for (uint64_t i = 0; i < m; i++) idx[i] = i;
+    std::sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) { return order[a] < order[b]; });
+    for (uint64_t i = 0; i < m; i++) {
+      if (keys) keys[i] = order[idx[i]];
+      if (scores) scores[i] = sc[idx[i]];
+    }
+    return NGIA_OK;
+  });
+}
+
+extern "C" int ngia_db_fast(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
+                            uint32_t* cluster, uint64_t* total_pairs) {
+  if (!ctx || !db || !cluster) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  if (identity < 1 || identity > 99) return fail(ctx, NGIA_ERR_ARG, "identity should be 1-99");
+  if (bands < 1 || bands > db->rep_bands) return fail(ctx, NGIA_ERR_ARG, "bad band count");
+  return guarded(ctx, [&] {
+    const uint32_t n = db->n;
+    DevBuf parent;
+    parent.ensure(4ull * std::max<uint32_t>(n, 1));
+    fill_u32(parent.as<uint32_t>(), 0xFFFFFFFFu, n, ctx->stream);
+    ngia_graph_stats st{};
+    graph_plain(ctx, db, identity, bands, &st, parent.as<uint32_t>());
+    std::vector<uint32_t> par(n);
+    d2h(par.data(), parent.p, 4ull * n, ctx->stream);
+    NGIA_HIP(hipStreamSynchronize(ctx->stream));
+    // fastClustering.hpp:338-347: parent[i] < i, resolve in ascending order
+    for (uint32_t i = 0; i < n; i++) cluster[i] = par[i] == 0xFFFFFFFFu ? i : cluster[par[i]];
+    if (total_pairs) *total_pairs = st.total_pairs;
+    return NGIA_OK;
+  });
+}
+
+// ------------------------------------------------------ host-pointer seams
+extern "C" int ngia_sketch(ngia_ctx* ctx, const uint32_t* packed, uint64_t n_words,
+                           const uint64_t* word_off, uint32_t n, uint32_t* sig) {
+  if (!ctx || !sig || (n && (!packed || !word_off))) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  if (n == 0) return NGIA_OK;
+  ngia_db* db = nullptr;
+  int rc = ngia_db_create(ctx, packed, n_words, word_off, nullptr, n, &db);
+  if (rc == NGIA_OK) rc = ngia_db_sketch(ctx, db);
+  if (rc == NGIA_OK) rc = ngia_db_get_sigs(ctx, db, sig);
+  ngia_db_destroy(db);
+  return rc;
+}
+
+extern "C" int ngia_bucket(ngia_ctx* ctx, const uint32_t* sig, uint32_t n, uint32_t* rep) {
+  if (!ctx || !sig || !rep) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  if (n == 0) return NGIA_OK;
+  return guarded(ctx, [&] {
+    DevBuf dsig, drep, scratch;
+    dsig.ensure(256ull * n);
+    drep.ensure(256ull * n);
+    scratch.ensure(bucket_scratch_bytes(n));
+    h2d(dsig.p, sig, 256ull * n, ctx->stream);
+    launch_bucket(dsig.as<uint32_t>(), n, drep.as<uint32_t>(), scratch.p, ctx->stream);
+    d2h(rep, drep.p, 256ull * n, ctx->stream);
+    NGIA_HIP(hipStreamSynchronize(ctx->stream));
+    return NGIA_OK;
+  });
+}
+
+extern "C" int ngia_align(ngia_ctx* ctx, const uint32_t* packed, uint64_t n_words,
+                          const uint64_t* word_off, uint32_t n_seqs, uint32_t* jobs,
+                          uint32_t count, float threshold) {
+  if (!ctx || !jobs || !packed || !word_off) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  for (uint64_t j = 0; j < count; j++)
+    if (jobs[3 * j] >= n_seqs || jobs[3 * j + 1] >= n_seqs)
+      return fail(ctx, NGIA_ERR_ARG, "job index out of range");
+  if (count == 0) return NGIA_OK;
+  ngia_db* db = nullptr;
+  int rc = ngia_db_create(ctx, packed, n_words, word_off, nullptr, n_seqs, &db);
+  if (rc != NGIA_OK) return rc;
+  rc = guarded(ctx, [&] {
+    hipStream_t s = ctx->stream;
+    DevBuf djobs;
+    djobs.ensure(12ull * count);
+    h2d(djobs.p, jobs, 12ull * count, s);
+    const uint32_t nw_max = (db->max_len + 31) / 32;
+    const uint64_t sw = align_plain_scratch_words(nw_max);
+    db->plain_scratch.ensure(4 * sw);
+    launch_align_plain(align_args(db, threshold), djobs.as<uint32_t>(), count, nw_max,
+                       db->plain_scratch.as<uint32_t>(), sw, s);
+    d2h(jobs, djobs.p, 12ull * count, s);
+    NGIA_HIP(hipStreamSynchronize(s));
+    return NGIA_OK;
+  });
+  ngia_db_destroy(db);
+  return rc;
+}
+
+extern "C" int ngia_build_graph(ngia_ctx* ctx, const uint32_t* packed, uint64_t n_words,
+                                const uint64_t* word_off, const uint32_t* lengths, uint32_t n,
+                                const uint32_t* rep_rows, uint32_t bands, uint32_t identity,
+                                uint32_t flags, ngia_edge** edges, uint64_t* n_edges,
+                                ngia_graph_stats* stats) {
+  if (!ctx || !edges || !n_edges || !rep_rows) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  *edges = nullptr;
+  *n_edges = 0;
+  ngia_db* db = nullptr;
+  int rc = ngia_db_create(ctx, packed, n_words, word_off, lengths, n, &db);
+  if (rc == NGIA_OK) rc = ngia_db_set_reps(ctx, db, rep_rows, bands);
+  if (rc == NGIA_OK) rc = ngia_db_graph(ctx, db, identity, bands, flags, stats);
+  if (rc == NGIA_OK) {
+    uint64_t m = 0;
+    ngia_db_get_edges(ctx, db, nullptr, 0, &m);
+    ngia_edge* e = (ngia_edge*)std::malloc(sizeof(ngia_edge) * std::max<uint64_t>(m, 1));
+    if (!e) rc = fail(ctx, NGIA_ERR_NOMEM, "host allocation failed");
+    if (rc == NGIA_OK) rc = ngia_db_get_edges(ctx, db, e, m, &m);
+    if (rc == NGIA_OK) {
+      *edges = e;
+      *n_edges = m;
+    } else {
+      std::free(e);
+    }
+  }
+  ngia_db_destroy(db);
+  return rc;
+}

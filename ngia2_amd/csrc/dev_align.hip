@@ -1,0 +1,451 @@
+// dev_align.hip -- banded bit-parallel LCS of candidate pairs on CDNA4.
+//
+// Semantics: kernel_dynamic (buildGraph.hpp:192-253).  For column block cb
+// (target characters 32cb..32cb+31) the reference updates rep words
+// [ws, we) = [max(cb - lsft/32, 0), ceil(min(32cb + rsft, R)/32)) in order,
+// with carry-in 0 into word ws and the carry out of word we-1 dropped; words
+// outside the range are frozen.  Because lsft is a multiple of 32, that range
+// is the window [cb - a, cb + c) with a = lsft/32, c = ceil(rsft/32), clipped
+// to [0, ceil(R/32)).  The window slides up one word per block, so a word
+// enters at the top as all ones (never touched before) and leaves at the
+// bottom frozen for good.  Words of the window outside [0, ceil(R/32)) are
+// "virtual": all-ones rows whose match mask is 0 -- such a row is a fixed
+// point of the update for either carry-in, so it absorbs the dropped carry at
+// the top and passes carry 0 at the bottom.  Target characters beyond T have
+// code 0 (match mask 0) and with carry-in 0 leave every word unchanged.
+// The kernels below all compute that recurrence exactly; they differ in how
+// pairs and words map onto lanes.
+//
+//   k_align_plain   one thread per job, one launch per band (the plain form)
+//   k_align_win<W>  one thread per pair, W window words in registers, the
+//                   per-lane match masks (Peq) in LDS, lane-interleaved
+//   k_align_wave<K> one wavefront per pair, K window words per lane, carry
+//                   propagated across lanes by a ballot carry-lookahead
+#include <algorithm>
+
+#include "dev_common.hpp"
+
+namespace ngia {
+
+// ------------------------------------------------------------- byte codes
+// codes[code_off[i] + j] = code of residue j of sequence i, 0 past the end.
+__global__ void k_codes(const uint32_t* __restrict__ packed, const uint64_t* __restrict__ word_off,
+                        const uint64_t* __restrict__ code_off, uint32_t n, uint8_t* __restrict__ codes) {
+  const uint32_t i = blockIdx.x;
+  if (i >= n) return;
+  const uint32_t* rec = packed + word_off[i];
+  const uint32_t nb = (rec[0] + 31) / 32;
+  uint32_t* out = (uint32_t*)(codes + code_off[i]);
+  for (uint32_t t = threadIdx.x; t < nb * 8; t += blockDim.x) {
+    const uint32_t b = t >> 3, q = t & 7;  // block b, chars 4q..4q+3
+    const uint32_t* pl = rec + 1 + 4 * b;
+    uint32_t w = 0;
+    #pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const uint32_t k = 4 * q + m;
+      uint32_t c = ((pl[0] >> k) & 1u) | (((pl[1] >> k) & 1u) << 1) | (((pl[2] >> k) & 1u) << 2) |
+                   (((pl[3] >> k) & 1u) << 3);
+      w |= c << (8 * m);
+    }
+    out[t] = w;
+  }
+}
+
+void launch_codes(const uint32_t* packed, const uint64_t* word_off, const uint64_t* code_off,
+                  uint32_t n, uint8_t* codes, hipStream_t stream) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_codes, dim3(n), dim3(64), 0, stream, packed, word_off, code_off, n, codes);
+  NGIA_HIP(hipGetLastError());
+}
+
+// The 15 match masks of one rep word from its 4 bit planes: mask[k] has bit
+// r set iff residue r of the word has code k (buildGraph.hpp:220-225).
+__device__ inline void peq_masks(uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3,
+                                 uint32_t m[16]) {
+  const uint32_t a0 = ~p0 & ~p1, a1 = p0 & ~p1, a2 = ~p0 & p1, a3 = p0 & p1;
+  const uint32_t b0 = ~p2 & ~p3, b1 = p2 & ~p3, b2 = ~p2 & p3, b3 = p2 & p3;
+  m[0] = 0;
+  m[1] = a1 & b0;  m[2] = a2 & b0;  m[3] = a3 & b0;
+  m[4] = a0 & b1;  m[5] = a1 & b1;  m[6] = a2 & b1;  m[7] = a3 & b1;
+  m[8] = a0 & b2;  m[9] = a1 & b2;  m[10] = a2 & b2; m[11] = a3 & b2;
+  m[12] = a0 & b3; m[13] = a1 & b3; m[14] = a2 & b3; m[15] = a3 & b3;
+}
+
+// One word-character step of the LCS row update (buildGraph.hpp:231-239):
+// row' = (row + (row & M) + cin) | (row & ~M), carry out returned in *cout.
+__device__ inline uint32_t lcs_step(uint32_t row, uint32_t M, uint32_t cin, uint32_t* cout) {
+  const uint32_t t0 = row & M;
+  const uint32_t s = __builtin_addc(row, t0, cin, cout);
+  uint32_t r = s | (row & ~M);  // one v_bitop3
+  // Keep the new row opaque: otherwise the next step's row & M is rebuilt
+  // from s and row & ~M, which costs an extra bitop3 per word-character.
+  asm("" : "+v"(r));
+  return r;
+}
+
+// ------------------------------------------------------------- plain form
+// One thread per job triplet, the whole band of any width; the row vector
+// lives in a lane-interleaved global scratch (rows of thread t at
+// scratch[w * stride + t]), the 16 match masks of the current rep word in a
+// lane-interleaved LDS table.  Grid-stride over the jobs.
+__global__ __launch_bounds__(64) void k_align_plain(AlignArgs A, uint32_t* __restrict__ jobs,
+                                                    uint32_t count, const uint32_t* count_ptr,
+                                                    uint32_t* __restrict__ scratch,
+                                                    uint32_t stride) {
+  __shared__ uint32_t peq[16 * 64];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t tid = blockIdx.x * 64 + lane;
+  if (count_ptr) count = *count_ptr;
+  for (uint32_t job = tid; job < count; job += stride) {
+    const uint32_t rep = jobs[3 * job], tar = jobs[3 * job + 1];
+    const uint32_t* rr = A.packed + A.word_off[rep];
+    const uint32_t R = rr[0];
+    const uint32_t T = A.packed[A.word_off[tar]];
+    if (length_rejects(R, T, A.thr)) continue;  // score keeps its initial 0
+    const uint32_t nw = (R + 31) / 32;
+    for (uint32_t w = 0; w < nw; w++) scratch[(uint64_t)w * stride + tid] = 0xFFFFFFFFu;
+    const Band bd = band_edges(R, T, A.thr);
+    const uint8_t* tc = A.codes + A.code_off[tar];
+    for (uint32_t i = 0; i < T; i += 32) {
+      int32_t js = (int32_t)i - (int32_t)bd.lsft;
+      if (js < 0) js = 0;
+      int32_t je = (int32_t)i + (int32_t)bd.rsft;
+      if (je > (int32_t)R) je = (int32_t)R;
+      const uint4 c0 = *(const uint4*)(tc + i), c1 = *(const uint4*)(tc + i + 16);
+      const uint32_t cw[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+      uint32_t carrys = 0;  // carry out of the previous word, bit k for char k
+      for (uint32_t j = (uint32_t)js; j < (uint32_t)je; j += 32) {
+        const uint32_t* pl = rr + 1 + (j / 32) * 4;
+        uint32_t m[16];
+        peq_masks(pl[0], pl[1], pl[2], pl[3], m);
+        #pragma unroll
+        for (int k = 0; k < 16; k++) peq[k * 64 + lane] = m[k];
+        uint32_t row = scratch[(uint64_t)(j / 32) * stride + tid];
+        uint32_t cout_bits = 0;
+        #pragma unroll
+        for (int k = 0; k < 32; k++) {
+          const uint32_t code = (cw[k >> 2] >> (8 * (k & 3))) & 0xFu;
+          const uint32_t M = peq[code * 64 + lane];
+          uint32_t co;
+          row = lcs_step(row, M, (carrys >> k) & 1u, &co);
+          cout_bits |= co << k;
+        }
+        carrys = cout_bits;
+        scratch[(uint64_t)(j / 32) * stride + tid] = row;
+      }
+    }
+    uint32_t lcs = 0;
+    for (uint32_t w = 0; w < nw; w++) lcs += 32u - __popc(scratch[(uint64_t)w * stride + tid]);
+    jobs[3 * job + 2] = lcs_score(lcs, R, T, A.thr);
+  }
+}
+
+static constexpr uint32_t kPlainBlocks = 4096;  // 262,144 threads
+
+uint64_t align_plain_scratch_words(uint32_t nw_max) {
+  return (uint64_t)kPlainBlocks * 64 * (nw_max ? nw_max : 1);
+}
+
+void launch_align_plain(const AlignArgs& a, uint32_t* jobs, uint32_t count, uint32_t nw_max,
+                        uint32_t* scratch, uint64_t scratch_words, hipStream_t stream,
+                        const uint32_t* count_ptr) {
+  if (count == 0) return;
+  uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)count + 63) / 64, kPlainBlocks);
+  if (blocks > kPlainBlocks) blocks = kPlainBlocks;
+  const uint32_t stride = blocks * 64;
+  if ((uint64_t)stride * nw_max > scratch_words)
+    throw HipError{NGIA_ERR_ARG, "plain alignment scratch too small"};
+  hipLaunchKernelGGL(k_align_plain, dim3(blocks), dim3(64), 0, stream, a, jobs, count, count_ptr,
+                     scratch, stride);
+  NGIA_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------ short window, thread/pair
+// Pair list entry: pairs[p] = (rep, tar); geometry computed by the binning
+// pass so that a + c == W for every pair of the list.
+template <int W>
+__global__ __launch_bounds__(64) void k_align_win(AlignArgs A, const uint2* __restrict__ pairs,
+                                                  const uint32_t* __restrict__ list,
+                                                  const uint32_t* __restrict__ off_ptr,
+                                                  const uint32_t* __restrict__ count_ptr,
+                                                  uint32_t* __restrict__ work,
+                                                  uint32_t* __restrict__ score) {
+  // Peq slots [W][16 codes][64 lanes] + staged target codes [8][64 lanes].
+  // Static so that its LDS address is a compile-time constant.
+  __shared__ uint32_t lds[W * 1024 + 8 * 64];
+  uint32_t* const cst = lds + W * 1024;
+  const uint32_t lane = threadIdx.x;
+  const uint32_t lane4 = lane * 4;
+  #pragma unroll
+  for (int s = 0; s < W; s++) lds[s * 1024 + lane] = 0;  // code 0 never matches
+  const uint32_t count = *count_ptr;
+  const uint32_t* const lst = list + (off_ptr ? *off_ptr : 0u);
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(work, 64u);
+    base = __shfl(base, 0);
+    if (base >= count) break;
+    const uint32_t idx = base + lane;
+    if (idx >= count) continue;
+    const uint32_t p = lst[idx];
+    const uint2 pr = pairs[p];
+    const uint32_t* rr = A.packed + A.word_off[pr.x];
+    const uint32_t R = rr[0];
+    const uint32_t T = A.packed[A.word_off[pr.y]];
+    const Band bd = band_edges(R, T, A.thr);
+    const int32_t a = (int32_t)(bd.lsft / 32);
+    const int32_t c = (int32_t)((bd.rsft + 31) / 32);
+    const int32_t nw = (int32_t)((R + 31) / 32);
+    const uint32_t nb = (T + 31) / 32;
+    const uint32_t* tc = (const uint32_t*)(A.codes + A.code_off[pr.y]);
+    // Window at block 0: relative word j <-> absolute j - a, register j, slot j.
+    uint32_t rows[W];
+    #pragma unroll
+    for (int j = 0; j < W; j++) {
+      rows[j] = 0xFFFFFFFFu;
+      const int32_t w = j - a;
+      uint32_t m[16];
+      if (w >= 0 && w < nw) {
+        const uint32_t* pl = rr + 1 + 4 * w;
+        peq_masks(pl[0], pl[1], pl[2], pl[3], m);
+      } else {
+        #pragma unroll
+        for (int k = 0; k < 16; k++) m[k] = 0;
+      }
+      #pragma unroll
+      for (int k = 1; k < 16; k++) lds[j * 1024 + k * 64 + lane] = m[k];
+    }
+    uint32_t lcs = 0;
+    for (uint32_t cb0 = 0; cb0 < nb; cb0 += W) {
+      #pragma unroll
+      for (int q = 0; q < W; q++) {
+        const uint32_t cb = cb0 + q;
+        if (cb >= nb) break;
+        // Stage this block's 32 target codes (4 per word) in LDS.
+        const uint4 c0 = *(const uint4*)(tc + 8 * cb), c1 = *(const uint4*)(tc + 8 * cb + 4);
+        cst[0 * 64 + lane] = c0.x; cst[1 * 64 + lane] = c0.y;
+        cst[2 * 64 + lane] = c0.z; cst[3 * 64 + lane] = c0.w;
+        cst[4 * 64 + lane] = c1.x; cst[5 * 64 + lane] = c1.y;
+        cst[6 * 64 + lane] = c1.z; cst[7 * 64 + lane] = c1.w;
+        for (int g = 0; g < 8; g++) {
+          const uint32_t cw = cst[g * 64 + lane];
+          #pragma unroll
+          for (int mm = 0; mm < 4; mm++) {
+            // byte address of Peq[code][lane] = code*256 + lane*4: v_perm takes
+            // byte 0 from lane4 (selector 0) and byte 1 from code byte mm of cw
+            // (selector 4+mm); bytes 2,3 are zero (selector 0x0C).
+            const uint32_t addr =
+                __builtin_amdgcn_perm(cw, lane4, 0x0C0C0000u | ((4u + (uint32_t)mm) << 8));
+            const uint32_t* pq = (const uint32_t*)((const char*)lds + addr);
+            uint32_t cin = 0;
+            #pragma unroll
+            for (int j = 0; j < W; j++) {
+              const int r = (q + j) % W;  // register and Peq slot of relative word j
+              rows[r] = lcs_step(rows[r], pq[r * 1024], cin, &cin);
+            }
+          }
+        }
+        // Slide: relative word 0 (register q) leaves, word cb + c enters there.
+        lcs += 32u - __popc(rows[q]);
+        rows[q] = 0xFFFFFFFFu;
+        const int32_t w = (int32_t)cb + c;
+        uint32_t m[16];
+        if (w < nw) {
+          const uint32_t* pl = rr + 1 + 4 * w;
+          peq_masks(pl[0], pl[1], pl[2], pl[3], m);
+        } else {
+          #pragma unroll
+          for (int k = 0; k < 16; k++) m[k] = 0;
+        }
+        #pragma unroll
+        for (int k = 1; k < 16; k++) lds[q * 1024 + k * 64 + lane] = m[k];
+      }
+    }
+    #pragma unroll
+    for (int j = 0; j < W; j++) lcs += 32u - __popc(rows[j]);
+    score[p] = lcs_score(lcs, R, T, A.thr);
+  }
+}
+
+// ------------------------------------------------ one wavefront per pair
+// Relative window word j = lane*K + m lives in rows[m] of that lane.  The
+// Peq table is a circular buffer of S = 64K word slots per code (plus K
+// mirrored slots so a lane's K consecutive reads never wrap); a word's masks
+// are written when it enters the band and cleared when it leaves, so slots
+// of words above the band read as 0.
+template <int K>
+__global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __restrict__ pairs,
+                                                   const uint32_t* __restrict__ list,
+                                                   const uint32_t* __restrict__ off_ptr,
+                                                   const uint32_t* __restrict__ count_ptr,
+                                                   uint32_t* __restrict__ work,
+                                                   uint32_t* __restrict__ score) {
+  constexpr uint32_t S = 64 * K;       // circular slots
+  constexpr uint32_t ST = S + K;       // row stride per code (with mirror)
+  extern __shared__ uint32_t peq[];    // [16][ST]
+  const uint32_t lane = threadIdx.x;
+  const uint32_t count = *count_ptr;
+  const uint32_t* const lst = list + (off_ptr ? *off_ptr : 0u);
+  for (;;) {
+    uint32_t it = 0;
+    if (lane == 0) it = atomicAdd(work, 1u);
+    it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+    if (it >= count) break;
+    const uint32_t p = lst[it];
+    const uint2 pr = pairs[p];
+    const uint32_t* rr = A.packed + A.word_off[pr.x];
+    const uint32_t R = rr[0];
+    const uint32_t T = A.packed[A.word_off[pr.y]];
+    const Band bd = band_edges(R, T, A.thr);
+    const int32_t a = (int32_t)(bd.lsft / 32);
+    const int32_t c = (int32_t)((bd.rsft + 31) / 32);
+    const int32_t nw = (int32_t)((R + 31) / 32);
+    const uint32_t nb = (T + 31) / 32;
+    const uint8_t* tc = A.codes + A.code_off[pr.y];
+    // Clear the table, then write the words in the band at block 0: [0, min(c, nw)).
+    for (uint32_t x = lane; x < 16 * ST; x += 64) peq[x] = 0;
+    __syncthreads();
+    auto slot_of = [&](int32_t w) -> uint32_t { return (uint32_t)(w + 64 * S) % S; };
+    auto write_word = [&](int32_t w, bool clear) {
+      // lanes 0..15 each write code `lane` of word w (code 0 stays 0)
+      if (lane >= 1 && lane < 16) {
+        uint32_t v = 0;
+        if (!clear) {
+          const uint32_t* pl = rr + 1 + 4 * w;
+          uint32_t m[16];
+          peq_masks(pl[0], pl[1], pl[2], pl[3], m);
+          #pragma unroll
+          for (int k = 1; k < 16; k++) v = (lane == (uint32_t)k) ? m[k] : v;
+        }
+        const uint32_t s = slot_of(w);
+        peq[lane * ST + s] = v;
+        if (s < K) peq[lane * ST + S + s] = v;
+      }
+    };
+    const int32_t top0 = c < nw ? c : nw;
+    for (int32_t w = 0; w < top0; w++) write_word(w, false);
+    __syncthreads();
+    uint32_t rows[K];
+    #pragma unroll
+    for (int m = 0; m < K; m++) rows[m] = 0xFFFFFFFFu;
+    uint32_t lcs = 0;
+    for (uint32_t cb = 0; cb < nb; cb++) {
+      const uint32_t vbase = slot_of((int32_t)cb - a + (int32_t)(lane * K));
+      const uint32_t* tcb = (const uint32_t*)(tc + 32 * cb);
+      for (int g = 0; g < 8; g++) {
+        const uint32_t cw = __builtin_amdgcn_readfirstlane(tcb[g]);
+        #pragma unroll
+        for (int mm = 0; mm < 4; mm++) {
+          const uint32_t code = (cw >> (8 * mm)) & 0xFu;
+          const uint32_t* pq = peq + code * ST + vbase;
+          uint32_t M[K], t0[K], s0[K];
+          uint32_t cy = 0;
+          #pragma unroll
+          for (int m = 0; m < K; m++) {
+            M[m] = pq[m];
+            t0[m] = rows[m] & M[m];
+            s0[m] = __builtin_addc(rows[m], t0[m], cy, &cy);
+          }
+          uint32_t all = s0[0];
+          #pragma unroll
+          for (int m = 1; m < K; m++) all &= s0[m];
+          const uint64_t G = __builtin_amdgcn_ballot_w64(cy != 0);
+          const uint64_t P = __builtin_amdgcn_ballot_w64(all == 0xFFFFFFFFu);
+          const uint64_t C = ((G | P) + G) ^ P;  // carry into each lane
+          uint32_t ci = __builtin_amdgcn_inverse_ballot_w64(C) ? 1u : 0u;
+          #pragma unroll
+          for (int m = 0; m < K; m++) {
+            const uint32_t s = __builtin_addc(s0[m], 0u, ci, &ci);
+            rows[m] = s | (rows[m] & ~M[m]);
+          }
+        }
+      }
+      // Slide the window one word up.
+      const uint32_t leaving = __builtin_amdgcn_readfirstlane(rows[0]);
+      lcs += 32u - __popc(leaving);
+      const uint32_t up = __shfl_down(rows[0], 1);
+      #pragma unroll
+      for (int m = 0; m < K - 1; m++) rows[m] = rows[m + 1];
+      rows[K - 1] = lane == 63 ? 0xFFFFFFFFu : up;
+      const int32_t wl = (int32_t)cb - a;      // leaving the band
+      const int32_t we = (int32_t)cb + c;      // entering the band at cb + 1
+      __syncthreads();
+      if (wl >= 0 && wl < nw) write_word(wl, true);
+      __syncthreads();
+      if (we < nw) write_word(we, false);
+      __syncthreads();
+    }
+    uint32_t part = 0;
+    #pragma unroll
+    for (int m = 0; m < K; m++) part += 32u - __popc(rows[m]);
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+    lcs += part;
+    if (lane == 0) score[p] = lcs_score(lcs, R, T, A.thr);
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------ launchers for the bins
+size_t win_lds_bytes(int W) { return (size_t)W * 4096 + 8 * 256; }
+size_t wave_lds_bytes(int K) { return (size_t)16 * (64 * K + K) * 4; }
+
+#define NGIA_WIN_CASE(Wv)                                                                  \
+  case Wv:                                                                                 \
+    hipLaunchKernelGGL(k_align_win<Wv>, dim3(blocks), dim3(64), 0, stream,                 \
+                       a, pairs, list, off_ptr, count_ptr, work, score);                   \
+    break;
+
+void launch_align_win(int W, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
+                      const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work, uint32_t* score,
+                      uint32_t blocks, hipStream_t stream) {
+  switch (W) {
+    NGIA_WIN_CASE(2)
+    NGIA_WIN_CASE(3)
+    NGIA_WIN_CASE(4)
+    NGIA_WIN_CASE(5)
+    NGIA_WIN_CASE(6)
+    NGIA_WIN_CASE(7)
+    NGIA_WIN_CASE(8)
+    default:
+      throw HipError{NGIA_ERR_ARG, "no window kernel for W=" + std::to_string(W)};
+  }
+  NGIA_HIP(hipGetLastError());
+}
+
+int align_kernels_setup(size_t lds_per_block) {
+  // K = 16, 32 need more than 64 KiB of LDS per workgroup; when the device
+  // refuses, windows wider than 64*8 words go to the plain kernel instead.
+  if (lds_per_block < wave_lds_bytes(32)) return 8;
+  if (hipFuncSetAttribute((const void*)k_align_wave<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)wave_lds_bytes(16)) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_align_wave<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)wave_lds_bytes(32)) != hipSuccess) {
+    (void)hipGetLastError();
+    return 8;
+  }
+  return kWaveKMax;
+}
+
+#define NGIA_WAVE_CASE(Kv)                                                                   \
+  case Kv:                                                                                   \
+    hipLaunchKernelGGL(k_align_wave<Kv>, dim3(blocks), dim3(64), wave_lds_bytes(Kv), stream, \
+                       a, pairs, list, off_ptr, count_ptr, work, score);                     \
+    break;
+
+void launch_align_wave(int K, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
+                       const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work, uint32_t* score,
+                       uint32_t blocks, hipStream_t stream) {
+  switch (K) {
+    NGIA_WAVE_CASE(1)
+    NGIA_WAVE_CASE(2)
+    NGIA_WAVE_CASE(4)
+    NGIA_WAVE_CASE(8)
+    NGIA_WAVE_CASE(16)
+    NGIA_WAVE_CASE(32)
+    default:
+      throw HipError{NGIA_ERR_ARG, "no wave kernel for K=" + std::to_string(K)};
+  }
+  NGIA_HIP(hipGetLastError());
+}
+
+}  // namespace ngia

@@ -1,0 +1,111 @@
+// dev_common.hpp -- shared declarations of the HIP side of the C-ABI library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "ngia2.h"
+
+namespace ngia {
+
+// Error carried from a HIP call site to the C-ABI boundary.
+struct HipError {
+  int code;
+  std::string msg;
+};
+
+#define NGIA_HIP(call)                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      throw ::ngia::HipError{NGIA_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+inline unsigned grid_for(uint64_t n, unsigned block) {
+  uint64_t g = (n + block - 1) / block;
+  return (unsigned)(g == 0 ? 1 : g);
+}
+
+// Band edges of the reference kernel (buildGraph.hpp:208-210), computed with
+// an explicit fused multiply-add: nvcc contracts `T - T*thr` into
+// fma(-T, thr, T) under its default -fmad=true.  Float -> uint32 conversion
+// saturates negative values to 0 on both GPU ISAs.
+struct Band {
+  uint32_t lsft, rsft;
+};
+__host__ __device__ inline uint32_t f2u_sat(float x) {
+  return x > 0.0f ? (x >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)x) : 0u;
+}
+__host__ __device__ inline Band band_edges(uint32_t R, uint32_t T, float thr) {
+  const float fT = (float)T, fR = (float)R;
+  Band b;
+  b.lsft = (f2u_sat(ceilf(fmaf(-fT, thr, fT))) + 31u) & ~31u;
+  b.rsft = f2u_sat(ceilf(fmaf(-fT, thr, fR))) + 32u;
+  return b;
+}
+// The reference kernel's length filter (buildGraph.hpp:201): the job is
+// skipped (score stays 0) when T < R*thr.
+__host__ __device__ inline bool length_rejects(uint32_t R, uint32_t T, float thr) {
+  return (float)T < (float)R * thr;
+}
+// Pass test and integer percent score (buildGraph.hpp:248-252).
+__host__ __device__ inline uint32_t lcs_score(uint32_t lcs, uint32_t R, uint32_t T, float thr) {
+  const uint32_t d = R + T - lcs;
+  return ((float)lcs > (float)d * thr) ? lcs * 100u / d : 0u;
+}
+
+// ---------------------------------------------------------------- launchers
+// All take device pointers and run on `stream`.
+
+// Band-major bucketing of n x 64 signatures into 64 x n representative rows.
+// scratch holds bucket_scratch_bytes(n) bytes.
+size_t bucket_scratch_bytes(uint32_t n);
+void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch,
+                   hipStream_t stream);
+
+// Byte-per-residue codes (code of residue j at byte code_off[i] + j, padding
+// to a multiple of 32 is code 0) built from the packed planes.
+void launch_codes(const uint32_t* packed, const uint64_t* word_off, const uint64_t* code_off,
+                  uint32_t n, uint8_t* codes, hipStream_t stream);
+
+// Geometry of one aligned pair.
+struct PairGeom {
+  uint32_t R, T;   // lengths
+  uint32_t a, c;   // window below / above: band words = [cb-a, cb+c)
+};
+
+// Alignment of a pair list: score[p] for pairs rep[p], tar[p], p < count.
+struct AlignArgs {
+  const uint32_t* packed;
+  const uint64_t* word_off;
+  const uint8_t* codes;
+  const uint64_t* code_off;
+  float thr;
+};
+// Plain per-band form: one thread per job triplet, any length.  The rep of
+// every job has at most nw_max words (ceil(R/32)).  With count_ptr set the
+// job count is read from device memory (count is then an upper bound).
+void launch_align_plain(const AlignArgs& a, uint32_t* jobs, uint32_t count, uint32_t nw_max,
+                        uint32_t* scratch, uint64_t scratch_words, hipStream_t stream,
+                        const uint32_t* count_ptr = nullptr);
+uint64_t align_plain_scratch_words(uint32_t nw_max);
+// Length-binned forms over a pair list (pairs[list[*off_ptr + i]], i <
+// *count_ptr); work is a zeroed work-queue counter; blocks is the persistent
+// grid.
+size_t win_lds_bytes(int W);
+size_t wave_lds_bytes(int K);
+void launch_align_win(int W, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
+                      const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work, uint32_t* score,
+                      uint32_t blocks, hipStream_t stream);
+void launch_align_wave(int K, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
+                       const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work, uint32_t* score,
+                       uint32_t blocks, hipStream_t stream);
+// Raise the dynamic-LDS limit of the kernels that need more than 64 KiB;
+// returns the largest K of k_align_wave<K> the device accepts.
+int align_kernels_setup(size_t lds_per_block);
+constexpr int kWinMax = 8;     // window widths 2..8 use k_align_win
+constexpr int kWaveKMax = 32;  // k_align_wave handles W <= 64*32
+
+}  // namespace ngia

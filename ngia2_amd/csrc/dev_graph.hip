@@ -1,0 +1,436 @@
+// dev_graph.hip -- candidate pairs, pair binning and edge emission for the
+// graph build (buildGraph.hpp:256-363) and fast mode (fastClustering.hpp:249-352).
+//
+// Every kernel here has a grid that depends only on host-known sizes (n,
+// bands, a persistent grid) and reads data-dependent counts from device
+// memory, so a whole round can be captured in a hipGraph.  Counters are
+// cleared with a fill kernel, never with hipMemsetAsync.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "dev_common.hpp"
+#include "dev_graph.hpp"
+
+namespace ngia {
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint64_t count) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+void fill_u32(uint32_t* p, uint32_t v, uint64_t count, hipStream_t s) {
+  if (count == 0) return;
+  uint64_t blocks = (count + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, v, count);
+  NGIA_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ scan
+// Exclusive scan of n uint32 into n+1 uint64 (out[n] = total), three passes
+// of 4096-element tiles.
+constexpr uint32_t kScanTile = 4096;
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n,
+                                                     uint64_t* __restrict__ sums) {
+  __shared__ uint64_t red[256];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  uint64_t s = 0;
+  for (uint32_t k = threadIdx.x; k < kScanTile; k += 256)
+    if (base + k < n) s += in[base + k];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(uint64_t* __restrict__ sums, uint32_t nb) {
+  // single block: exclusive scan of nb block sums in chunks of 1024
+  __shared__ uint64_t sh[1024];
+  uint64_t carry = 0;
+  for (uint32_t c0 = 0; c0 < nb; c0 += 1024) {
+    const uint32_t i = c0 + threadIdx.x;
+    const uint64_t v = i < nb ? sums[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+      const uint64_t t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) sums[i] = carry + sh[threadIdx.x] - v;
+    const uint64_t tot = sh[1023];
+    __syncthreads();
+    carry += tot;
+  }
+  if (threadIdx.x == 0) sums[nb] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t n,
+                                                    const uint64_t* __restrict__ sums,
+                                                    uint64_t* __restrict__ out, uint32_t nb) {
+  __shared__ uint64_t sh[256];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  // each thread owns 16 consecutive elements
+  const uint64_t mine = base + (uint64_t)threadIdx.x * 16;
+  uint32_t v[16];
+  uint64_t s = 0;
+  #pragma unroll
+  for (int k = 0; k < 16; k++) {
+    v[k] = (mine + k < n) ? in[mine + k] : 0u;
+    s += v[k];
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t off = 1; off < 256; off <<= 1) {
+    const uint64_t t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  uint64_t run = sums[blockIdx.x] + sh[threadIdx.x] - s;
+  #pragma unroll
+  for (int k = 0; k < 16; k++) {
+    if (mine + k < n) out[mine + k] = run;
+    run += v[k];
+  }
+  if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = sums[nb];
+}
+
+uint64_t scan_scratch_words(uint32_t n) { return (n + kScanTile - 1) / kScanTile + 2; }
+
+void scan_u32(const uint32_t* in, uint32_t n, uint64_t* out, uint64_t* sums, hipStream_t s) {
+  const uint32_t nb = (n + kScanTile - 1) / kScanTile;
+  if (nb == 0) {
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, 0u);
+    // out[0] = 0 via a one-element copy of sums[0] (= 0 after k_scan_sums)
+    NGIA_HIP(hipMemcpyAsync(out, sums, 8, hipMemcpyDeviceToDevice, s));
+    return;
+  }
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in, n, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nb);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, in, n, sums, out, nb);
+  NGIA_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------- predicate
+// Candidate predicate of buildGraph.hpp:262-266: rep[i] != i and
+// (float)len[i] >= (float)len[rep[i]] * thr.
+__device__ inline bool is_job(uint32_t r, uint32_t i, const uint32_t* len, float thr) {
+  return r != i && (float)len[i] >= (float)len[r] * thr;
+}
+
+// ------------------------------------------------------------- plain form
+// Per band: flags, scan, then job triplets in ascending i (buildGraph.hpp:278-321).
+__global__ void k_band_flags(const uint32_t* __restrict__ rep, const uint32_t* __restrict__ len,
+                             uint32_t n, float thr, const uint32_t* __restrict__ parent,
+                             uint32_t* __restrict__ flags) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t r = rep[i];
+  // fast mode (fastClustering.hpp:256-258) has no length filter, only "unclustered"
+  flags[i] = parent ? (uint32_t)(r != i && parent[i] == 0xFFFFFFFFu) : (uint32_t)is_job(r, i, len, thr);
+}
+
+__global__ void k_band_jobs(const uint32_t* __restrict__ rep, const uint32_t* __restrict__ flags,
+                            const uint64_t* __restrict__ pos, uint32_t n,
+                            uint32_t* __restrict__ jobs) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flags[i]) return;
+  const uint64_t j = pos[i];
+  jobs[3 * j] = rep[i];
+  jobs[3 * j + 1] = i;
+  jobs[3 * j + 2] = 0;
+}
+
+void band_jobs(const uint32_t* rep, const uint32_t* len, uint32_t n, float thr,
+               const uint32_t* parent, uint32_t* flags, uint64_t* pos, uint64_t* sums,
+               uint32_t* jobs, hipStream_t s) {
+  hipLaunchKernelGGL(k_band_flags, dim3(grid_for(n, 256)), dim3(256), 0, s, rep, len, n, thr,
+                     parent, flags);
+  NGIA_HIP(hipGetLastError());
+  scan_u32(flags, n, pos, sums, s);
+  hipLaunchKernelGGL(k_band_jobs, dim3(grid_for(n, 256)), dim3(256), 0, s, rep, flags, pos, n, jobs);
+  NGIA_HIP(hipGetLastError());
+}
+
+// Append the passing jobs of one band to the raw edge list.
+__global__ void k_jobs_to_edges(const uint32_t* __restrict__ jobs, uint32_t count,
+                                uint64_t* __restrict__ ekeys, uint32_t* __restrict__ escores,
+                                unsigned long long* __restrict__ ecount) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const uint32_t sc = jobs[3 * j + 2];
+  if (sc == 0) return;
+  const unsigned long long at = atomicAdd(ecount, 1ull);
+  ekeys[at] = ((uint64_t)jobs[3 * j] << 32) | jobs[3 * j + 1];
+  escores[at] = sc;
+}
+
+void jobs_to_edges(const uint32_t* jobs, uint32_t count, uint64_t* ekeys, uint32_t* escores,
+                   unsigned long long* ecount, hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_jobs_to_edges, dim3(grid_for(count, 256)), dim3(256), 0, s, jobs, count,
+                     ekeys, escores, ecount);
+  NGIA_HIP(hipGetLastError());
+}
+
+// Fast mode: parent[tar] = rep for each passing job of the band.
+__global__ void k_fast_apply(const uint32_t* __restrict__ jobs, uint32_t count,
+                             uint32_t* __restrict__ parent) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  if (jobs[3 * j + 2] > 0) parent[jobs[3 * j + 1]] = jobs[3 * j];
+}
+
+void fast_apply(const uint32_t* jobs, uint32_t count, uint32_t* parent, hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_fast_apply, dim3(grid_for(count, 256)), dim3(256), 0, s, jobs, count,
+                     parent);
+  NGIA_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------- batched form
+// One wavefront per target i; lane b < bands reads rep[b][i].  The distinct
+// representatives passing the candidate predicate are kept in order of their
+// first band.  Count pass (pairs == nullptr) writes cnt[i] and adds the
+// candidate count with duplicates to *total; emit pass writes the pairs at
+// off[i].
+__global__ __launch_bounds__(256) void k_pair_scan(const uint32_t* __restrict__ rep,
+                                                   const uint32_t* __restrict__ len, uint32_t n,
+                                                   uint32_t bands, float thr,
+                                                   uint32_t* __restrict__ cnt,
+                                                   unsigned long long* __restrict__ total,
+                                                   const uint64_t* __restrict__ off,
+                                                   uint2* __restrict__ pairs) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (i >= n) return;
+  uint32_t r = 0xFFFFFFFFu;
+  bool valid = false;
+  if (lane < bands) {
+    r = rep[(uint64_t)lane * n + i];
+    valid = is_job(r, i, len, thr);
+  }
+  const uint64_t vm = __builtin_amdgcn_ballot_w64(valid);
+  uint64_t mask = vm, keep = 0;
+  while (mask) {
+    const uint32_t b0 = (uint32_t)__builtin_ctzll(mask);
+    const uint32_t r0 = __builtin_amdgcn_readlane(r, b0);
+    keep |= 1ull << b0;
+    mask &= ~__builtin_amdgcn_ballot_w64(valid && r == r0);
+  }
+  if (!pairs) {
+    if (lane == 0) {
+      cnt[i] = (uint32_t)__builtin_popcountll(keep);
+      if (vm) atomicAdd(total, (unsigned long long)__builtin_popcountll(vm));
+    }
+    return;
+  }
+  if ((keep >> lane) & 1ull) {
+    const uint32_t rank = (uint32_t)__builtin_popcountll(keep & ((1ull << lane) - 1ull));
+    pairs[off[i] + rank] = make_uint2(r, i);
+  }
+}
+
+void pair_count(const uint32_t* rep, const uint32_t* len, uint32_t n, uint32_t bands, float thr,
+                uint32_t* cnt, unsigned long long* total, hipStream_t s) {
+  hipLaunchKernelGGL(k_pair_scan, dim3((n + 3) / 4), dim3(256), 0, s, rep, len, n, bands, thr, cnt,
+                     total, (const uint64_t*)nullptr, (uint2*)nullptr);
+  NGIA_HIP(hipGetLastError());
+}
+
+void pair_emit(const uint32_t* rep, const uint32_t* len, uint32_t n, uint32_t bands, float thr,
+               const uint64_t* off, uint2* pairs, hipStream_t s) {
+  hipLaunchKernelGGL(k_pair_scan, dim3((n + 3) / 4), dim3(256), 0, s, rep, len, n, bands, thr,
+                     (uint32_t*)nullptr, (unsigned long long*)nullptr, off, pairs);
+  NGIA_HIP(hipGetLastError());
+}
+
+// Bin of a pair by its band window: W = a + c words.  Bins 2..kWinMax go to
+// k_align_win<W>; larger windows to k_align_wave<K> with the smallest K in
+// {1,2,4,...,32} with 64K >= W; T == 0 scores 0 at once (its LCS is 0, so the
+// pass test fails); anything wider goes to the plain kernel.
+__device__ inline uint32_t pair_bin(uint32_t R, uint32_t T, float thr, uint32_t kmax,
+                                    uint64_t* tiles) {
+  if (length_rejects(R, T, thr) || T == 0) {
+    *tiles = 0;
+    return kBinZero;
+  }
+  const Band bd = band_edges(R, T, thr);
+  const uint32_t a = bd.lsft / 32, c = (bd.rsft + 31) / 32, W = a + c;
+  // reference tiles: words of [max(32cb - lsft, 0), min(32cb + rsft, R)) per block
+  uint64_t t = 0;
+  const uint32_t nb = (T + 31) / 32;
+  for (uint32_t cb = 0; cb < nb; cb++) {
+    int32_t js = (int32_t)(32 * cb) - (int32_t)bd.lsft;
+    if (js < 0) js = 0;
+    int32_t je = (int32_t)(32 * cb) + (int32_t)bd.rsft;
+    if (je > (int32_t)R) je = (int32_t)R;
+    if (je > js) t += ((uint32_t)(je - js) + 31) / 32;
+  }
+  *tiles = t;
+  if (W <= (uint32_t)kWinMax) return W;  // W >= 2 whenever T > 0
+  for (uint32_t k = 0, K = 1; K <= kmax; k++, K <<= 1)
+    if (W <= 64u * (uint32_t)K) return kBinWave0 + k;
+  return kBinPlain;
+}
+
+__global__ void k_bin_count(const uint2* __restrict__ pairs, const uint64_t* __restrict__ npairs,
+                            const uint32_t* __restrict__ len, float thr, uint32_t kmax,
+                            uint32_t* __restrict__ bin_of, uint32_t* __restrict__ bin_cnt,
+                            unsigned long long* __restrict__ tiles_total,
+                            uint32_t* __restrict__ score) {
+  const uint64_t P = *npairs;
+  __shared__ uint32_t local[kBins];
+  __shared__ unsigned long long ltiles;
+  if (threadIdx.x < kBins) local[threadIdx.x] = 0;
+  if (threadIdx.x == 0) ltiles = 0;
+  __syncthreads();
+  uint64_t tsum = 0;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint2 pr = pairs[p];
+    uint64_t t;
+    const uint32_t b = pair_bin(len[pr.x], len[pr.y], thr, kmax, &t);
+    bin_of[p] = b;
+    tsum += t;
+    if (b == kBinZero) score[p] = 0;
+    atomicAdd(&local[b], 1u);
+  }
+  atomicAdd(&ltiles, (unsigned long long)tsum);
+  __syncthreads();
+  if (threadIdx.x < kBins && local[threadIdx.x]) atomicAdd(&bin_cnt[threadIdx.x], local[threadIdx.x]);
+  if (threadIdx.x == 0 && ltiles) atomicAdd(tiles_total, ltiles);
+}
+
+// bin_off[b] = exclusive prefix of bin_cnt; also resets the scatter cursors.
+__global__ void k_bin_offsets(const uint32_t* __restrict__ bin_cnt, uint32_t* __restrict__ bin_off,
+                              uint32_t* __restrict__ cursor) {
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int b = 0; b < kBins; b++) {
+      bin_off[b] = s;
+      cursor[b] = 0;
+      s += bin_cnt[b];
+    }
+    bin_off[kBins] = s;
+  }
+}
+
+__global__ void k_bin_scatter(const uint64_t* __restrict__ npairs, const uint32_t* __restrict__ bin_of,
+                              const uint32_t* __restrict__ bin_off, uint32_t* __restrict__ cursor,
+                              uint32_t* __restrict__ list) {
+  const uint64_t P = *npairs;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t b = bin_of[p];
+    const uint32_t at = atomicAdd(&cursor[b], 1u);
+    list[bin_off[b] + at] = (uint32_t)p;
+  }
+}
+
+void bin_pairs(const uint2* pairs, const uint64_t* npairs, const uint32_t* len, float thr,
+               uint32_t* bin_of, uint32_t* bin_cnt, uint32_t* bin_off, uint32_t* cursor,
+               uint32_t* list, unsigned long long* tiles_total, uint32_t* score, uint32_t kmax,
+               uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_count, dim3(grid), dim3(256), 0, s, pairs, npairs, len, thr, kmax, bin_of,
+                     bin_cnt, tiles_total, score);
+  hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, bin_cnt, bin_off, cursor);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(grid), dim3(256), 0, s, npairs, bin_of, bin_off, cursor,
+                     list);
+  NGIA_HIP(hipGetLastError());
+}
+
+// Pairs of the plain-kernel bin as job triplets, scores copied back after.
+__global__ void k_bin_to_jobs(const uint2* __restrict__ pairs, const uint32_t* __restrict__ list,
+                              const uint32_t* __restrict__ bin_off, const uint32_t* __restrict__ bin_cnt,
+                              uint32_t* __restrict__ jobs) {
+  const uint32_t cnt = bin_cnt[kBinPlain], off = bin_off[kBinPlain];
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
+    const uint2 pr = pairs[list[off + j]];
+    jobs[3 * j] = pr.x;
+    jobs[3 * j + 1] = pr.y;
+    jobs[3 * j + 2] = 0;
+  }
+}
+__global__ void k_jobs_to_scores(const uint32_t* __restrict__ list, const uint32_t* __restrict__ bin_off,
+                                 const uint32_t* __restrict__ bin_cnt, const uint32_t* __restrict__ jobs,
+                                 uint32_t* __restrict__ score) {
+  const uint32_t cnt = bin_cnt[kBinPlain], off = bin_off[kBinPlain];
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x)
+    score[list[off + j]] = jobs[3 * j + 2];
+}
+void plain_bin_jobs(const uint2* pairs, const uint32_t* list, const uint32_t* bin_off,
+                    const uint32_t* bin_cnt, uint32_t* jobs, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_to_jobs, dim3(64), dim3(256), 0, s, pairs, list, bin_off, bin_cnt, jobs);
+  NGIA_HIP(hipGetLastError());
+}
+void plain_bin_scores(const uint32_t* list, const uint32_t* bin_off, const uint32_t* bin_cnt,
+                      const uint32_t* jobs, uint32_t* score, hipStream_t s) {
+  hipLaunchKernelGGL(k_jobs_to_scores, dim3(64), dim3(256), 0, s, list, bin_off, bin_cnt, jobs,
+                     score);
+  NGIA_HIP(hipGetLastError());
+}
+
+// Passing pairs -> raw edge list (order of the pair list, which is by target).
+__global__ void k_pairs_to_edges(const uint2* __restrict__ pairs, const uint64_t* __restrict__ npairs,
+                                 const uint32_t* __restrict__ score, uint64_t* __restrict__ ekeys,
+                                 uint32_t* __restrict__ escores,
+                                 unsigned long long* __restrict__ ecount) {
+  const uint64_t P = *npairs;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t sc = score[p];
+    if (sc == 0) continue;
+    const uint2 pr = pairs[p];
+    const unsigned long long at = atomicAdd(ecount, 1ull);
+    ekeys[at] = ((uint64_t)pr.x << 32) | pr.y;
+    escores[at] = sc;
+  }
+}
+
+void pairs_to_edges(const uint2* pairs, const uint64_t* npairs, const uint32_t* score,
+                    uint64_t* ekeys, uint32_t* escores, unsigned long long* ecount, uint32_t grid,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_pairs_to_edges, dim3(grid), dim3(256), 0, s, pairs, npairs, score, ekeys,
+                     escores, ecount);
+  NGIA_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------- finalize
+// Sort the raw edges by (u,v) and drop duplicates (buildGraph.hpp:355-362):
+// the score is a function of (u,v), so unique by key equals the reference's
+// unique over (u,v,w) tuples.
+size_t finalize_scratch_bytes(uint64_t m) {
+  size_t a = 0, b = 0;
+  (void)rocprim::radix_sort_pairs(nullptr, a, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                  (uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)m, 0, 63);
+  (void)rocprim::unique_by_key(nullptr, b, (uint64_t*)nullptr, (uint32_t*)nullptr,
+                               (uint64_t*)nullptr, (uint32_t*)nullptr,
+                               (unsigned long long*)nullptr, (size_t)m);
+  return (a > b ? a : b) + 256;
+}
+
+void finalize_edges(uint64_t* keys, uint32_t* scores, uint64_t m, uint64_t* keys_tmp,
+                    uint32_t* scores_tmp, void* temp, size_t temp_bytes,
+                    unsigned long long* d_unique, hipStream_t s) {
+  if (m == 0) {
+    fill_u32((uint32_t*)d_unique, 0, 2, s);
+    return;
+  }
+  size_t tb = temp_bytes;
+  NGIA_HIP(rocprim::radix_sort_pairs(temp, tb, keys, keys_tmp, scores, scores_tmp, (size_t)m, 0, 63,
+                                     s));
+  tb = temp_bytes;
+  NGIA_HIP(rocprim::unique_by_key(temp, tb, keys_tmp, scores_tmp, keys, scores, d_unique, (size_t)m,
+                                  rocprim::equal_to<uint64_t>(), s));
+}
+
+}  // namespace ngia

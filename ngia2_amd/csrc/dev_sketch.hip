@@ -1,0 +1,168 @@
+// dev_sketch.hip -- minhash sketch (makedb.hpp:224-247) and band bucketing
+// (makedb.hpp:289-364) on the device.
+//
+// Sketch mapping: one wavefront per sequence, lane k computes signature k.
+// The 8-mer is the same for every lane, so it is built on the scalar unit
+// from the sequence's byte codes (scalar loads, four codes per dword); the
+// 64 hash functions run one per lane.  The first two steps of the hash fold:
+// with h = kmer ^ seed_k,  (h ^ 61) ^ (h >> 16) =
+//   (kmer ^ (kmer >> 16)) ^ (seed_k ^ 61 ^ (seed_k >> 16)),
+// a wave-uniform term xor a per-lane constant, because a logical right shift
+// distributes over xor.  Output is sequence-major: lane k of sequence i
+// writes sig[64*i + k], one 256-byte coalesced store per wavefront.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "dev_common.hpp"
+
+namespace ngia {
+
+__global__ __launch_bounds__(256) void k_sketch(const uint8_t* __restrict__ codes,
+                                                const uint64_t* __restrict__ code_off,
+                                                const uint32_t* __restrict__ lengths,
+                                                uint32_t n, uint32_t* __restrict__ sig) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t seq = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (seq >= n) return;
+  const uint32_t L = __builtin_amdgcn_readfirstlane(lengths[seq]);
+  const uint32_t* cw = (const uint32_t*)(codes + __builtin_amdgcn_readfirstlane(code_off[seq]));
+  const uint32_t seed = lane * 0x9e3779b1u + 0x85ebca6bu;
+  const uint32_t s1 = seed ^ 61u ^ (seed >> 16);
+  uint32_t mn = 0xFFFFFFFFu;
+  uint32_t kmer = 0;
+  // Hashing starts at position 7, or at the last position of a read shorter
+  // than 8 (makedb.hpp:234-235).
+  const uint32_t first = L >= 8 ? 7u : (L ? L - 1 : 0u);
+  auto hash = [&](uint32_t k1) {
+    uint32_t h = k1 ^ s1;
+    h = h + (h << 3);
+    h = h ^ (h >> 4);
+    h = h * 0x27d4eb2du;
+    h = h ^ (h >> 15);
+    h = (h ^ (h >> 8)) * 0x9e3779b1u;
+    h = h ^ (h >> 14);
+    mn = h < mn ? h : mn;
+  };
+  uint32_t j = 0;
+  // Prefix that only feeds the k-mer.
+  for (; j < first; j++) {
+    const uint32_t w = __builtin_amdgcn_readfirstlane(cw[j >> 2]);
+    kmer = (kmer << 4) + ((w >> (8 * (j & 3))) & 0xFu);
+  }
+  // Head up to a 4-aligned position, then four positions per code dword.
+  for (; j < L && (j & 3); j++) {
+    const uint32_t w = __builtin_amdgcn_readfirstlane(cw[j >> 2]);
+    kmer = (kmer << 4) + ((w >> (8 * (j & 3))) & 0xFu);
+    hash(kmer ^ (kmer >> 16));
+  }
+  for (; j + 4 <= L; j += 4) {
+    const uint32_t w = __builtin_amdgcn_readfirstlane(cw[j >> 2]);
+    #pragma unroll
+    for (int m = 0; m < 4; m++) {
+      kmer = (kmer << 4) + ((w >> (8 * m)) & 0xFu);
+      hash(kmer ^ (kmer >> 16));
+    }
+  }
+  for (; j < L; j++) {
+    const uint32_t w = __builtin_amdgcn_readfirstlane(cw[j >> 2]);
+    kmer = (kmer << 4) + ((w >> (8 * (j & 3))) & 0xFu);
+    hash(kmer ^ (kmer >> 16));
+  }
+  sig[(uint64_t)seq * 64 + lane] = mn;
+}
+
+void launch_sketch(const uint8_t* codes, const uint64_t* code_off, const uint32_t* lengths,
+                   uint32_t n, uint32_t* sig, hipStream_t stream) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_sketch, dim3((n + 3) / 4), dim3(256), 0, stream, codes, code_off, lengths,
+                     n, sig);
+  NGIA_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------- bucketing
+// Keys (band << 32 | sig) in band-major order with values i ascending inside
+// each band; a stable radix sort on the low 38 bits then yields, per band,
+// the (sig, i)-lexicographic order the reference's LSD radix produces.
+__global__ __launch_bounds__(256) void k_bucket_keys(const uint32_t* __restrict__ sig, uint32_t n,
+                                                     uint64_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ vals) {
+  // 64 sequences x 64 bands tile through LDS: coalesced reads of the
+  // sequence-major signatures, coalesced writes of band-major keys.
+  __shared__ uint32_t tile[64][65];
+  const uint32_t i0 = blockIdx.x * 64;
+  const uint32_t t = threadIdx.x;
+  for (uint32_t e = t; e < 64 * 64; e += 256) {
+    const uint32_t si = e >> 6, b = e & 63;
+    tile[si][b] = (i0 + si < n) ? sig[(uint64_t)(i0 + si) * 64 + b] : 0u;
+  }
+  __syncthreads();
+  for (uint32_t e = t; e < 64 * 64; e += 256) {
+    const uint32_t b = e >> 6, si = e & 63;
+    const uint32_t i = i0 + si;
+    if (i < n) {
+      keys[(uint64_t)b * n + i] = ((uint64_t)b << 32) | tile[si][b];
+      vals[(uint64_t)b * n + i] = i;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bucket_rep(const uint64_t* __restrict__ keys,
+                                                    const uint32_t* __restrict__ vals,
+                                                    uint64_t total, uint32_t n,
+                                                    uint32_t* __restrict__ rep) {
+  const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= total) return;
+  const uint64_t k = keys[p];
+  const uint32_t i = vals[p];
+  const uint32_t b = (uint32_t)(k >> 32);
+  const bool same = p > 0 && keys[p - 1] == k;
+  rep[(uint64_t)b * n + i] = same ? vals[p - 1] : i;
+}
+
+namespace {
+struct BucketLayout {
+  size_t keys_in, keys_out, vals_in, vals_out, temp, temp_bytes, total;
+};
+BucketLayout bucket_layout(uint32_t n) {
+  const uint64_t m = (uint64_t)n * 64;
+  size_t temp_bytes = 0;
+  (void)rocprim::radix_sort_pairs(nullptr, temp_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                  (uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)m, 0, 38);
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  BucketLayout L;
+  L.keys_in = 0;
+  L.keys_out = L.keys_in + al(8 * m);
+  L.vals_in = L.keys_out + al(8 * m);
+  L.vals_out = L.vals_in + al(4 * m);
+  L.temp = L.vals_out + al(4 * m);
+  L.temp_bytes = temp_bytes;
+  L.total = L.temp + al(temp_bytes);
+  return L;
+}
+}  // namespace
+
+size_t bucket_scratch_bytes(uint32_t n) { return bucket_layout(n).total; }
+
+void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch,
+                   hipStream_t stream) {
+  if (n == 0) return;
+  const BucketLayout L = bucket_layout(n);
+  char* s = (char*)scratch;
+  uint64_t* keys_in = (uint64_t*)(s + L.keys_in);
+  uint64_t* keys_out = (uint64_t*)(s + L.keys_out);
+  uint32_t* vals_in = (uint32_t*)(s + L.vals_in);
+  uint32_t* vals_out = (uint32_t*)(s + L.vals_out);
+  const uint64_t m = (uint64_t)n * 64;
+  hipLaunchKernelGGL(k_bucket_keys, dim3((n + 63) / 64), dim3(256), 0, stream, sig, n, keys_in,
+                     vals_in);
+  NGIA_HIP(hipGetLastError());
+  size_t tb = L.temp_bytes;
+  NGIA_HIP(rocprim::radix_sort_pairs(s + L.temp, tb, keys_in, keys_out, vals_in, vals_out,
+                                     (size_t)m, 0, 38, stream));
+  hipLaunchKernelGGL(k_bucket_rep, dim3(grid_for(m, 256)), dim3(256), 0, stream, keys_out,
+                     vals_out, m, n, rep);
+  NGIA_HIP(hipGetLastError());
+}
+
+}  // namespace ngia

@@ -1,0 +1,241 @@
+"""Parity of the HIP path (through the C-ABI) with the oracle, bit for bit.
+
+Each test prints the case it runs (flushed) before touching the GPU.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import ngia2_amd as ng
+import oracle_lib as orc
+from conftest import AMINO, family_fasta
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c1db(tmp_path_factory):
+    """A config-1 style set (scaled to 40 families) as a packed DB built by the oracle."""
+    d = tmp_path_factory.mktemp("c1")
+    fa, db = str(d / "c1.fa"), str(d / "c1.bin")
+    p = ng.gen_config(1, scale=0.2, seed=7)
+    ng.gen_fasta(p, fa)
+    assert orc.makedb(fa, db) == 0
+    return fa, db, orc.read_db_numpy(db)
+
+
+@pytest.fixture(scope="module")
+def longdb(tmp_path_factory):
+    """Long, close families (window widths far beyond the per-lane kernel)."""
+    d = tmp_path_factory.mktemp("long")
+    fa, db = str(d / "long.fa"), str(d / "long.bin")
+    rng = np.random.default_rng(11)
+    lengths = [900, 1500, 2600, 4100, 6000, 9000, 700, 300, 64, 31]
+    family_fasta(fa, rng, lengths, members=4, sub=0.03, indel=0.01)
+    assert orc.makedb(fa, db) == 0
+    return fa, db, orc.read_db_numpy(db)
+
+
+def edge_case_records():
+    """Packed records of awkward sequences: empty, shorter than 8, 32-boundaries,
+    unknown letters, lowercase."""
+    rng = np.random.default_rng(3)
+    seqs = [b"", b"A", b"ACDEFG", b"ACDEFGH", b"ACDEFGHI", b"XXXXXXXXXX", b"acdefghiklmn",
+            b"BJOZ*-AC"]
+    for L in (31, 32, 33, 63, 64, 65, 95, 96, 97, 500, 2049):
+        seqs.append(bytes(AMINO[i] for i in rng.integers(20, size=L)))
+    recs = [orc.pack(s) for s in seqs]
+    off = np.cumsum([0] + [len(r) for r in recs[:-1]]).astype(np.uint64)
+    return np.concatenate(recs).astype(np.uint32), off, seqs
+
+
+def test_sketch_edge_cases(ctx):
+    packed, off, _ = edge_case_records()
+    say("sketch edge cases", len(off))
+    got = ctx.sketch(packed, off)
+    want = orc.sketch_all(packed, off)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_sketch_c1(ctx, c1db):
+    _, _, d = c1db
+    say("sketch c1", d["n"])
+    got = ctx.sketch(d["packed"], d["word_off"])
+    want = orc.sketch_all(d["packed"], d["word_off"])
+    np.testing.assert_array_equal(got, want)
+
+
+def test_bucket_c1(ctx, c1db):
+    _, _, d = c1db
+    sig = orc.sketch_all(d["packed"], d["word_off"])
+    say("bucket c1", d["n"])
+    np.testing.assert_array_equal(ctx.bucket(sig), orc.bucket(sig))
+    # the rep rows the oracle wrote into the DB file are the same rows
+    np.testing.assert_array_equal(ctx.bucket(sig), d["rep"])
+
+
+def test_bucket_collisions(ctx):
+    # heavy signature collisions: few distinct values per band
+    rng = np.random.default_rng(5)
+    sig = rng.integers(0, 7, size=(3000, 64)).astype(np.uint32)
+    sig[:, 5] = 0xFFFFFFFF
+    say("bucket collisions", sig.shape)
+    np.testing.assert_array_equal(ctx.bucket(sig), orc.bucket(sig))
+
+
+@pytest.mark.parametrize("identity", [1, 50, 70, 90, 97, 99])
+def test_align_random_pairs(ctx, c1db, identity):
+    _, _, d = c1db
+    n = d["n"]
+    rng = np.random.default_rng(identity)
+    jobs = np.zeros((4000, 3), dtype=np.uint32)
+    jobs[:, 0] = rng.integers(0, n, size=4000)
+    jobs[:, 1] = rng.integers(0, n, size=4000)
+    # include same-family neighbours: use the DB's band-0 chains
+    r0 = d["rep"][0]
+    t = np.nonzero(r0 != np.arange(n))[0][:2000]
+    jobs[:len(t), 0] = r0[t]
+    jobs[:len(t), 1] = t
+    thr = np.float32(identity) / np.float32(100)
+    say("align plain kernel, identity", identity)
+    got = ctx.align(d["packed"], d["word_off"], jobs, float(thr))
+    want = orc.align_jobs(d["packed"], d["word_off"], jobs, float(thr))
+    np.testing.assert_array_equal(got, want)
+    assert (want[:, 2] > 0).any() or identity >= 90
+
+
+@pytest.mark.parametrize("identity", [50, 70, 90])
+@pytest.mark.parametrize("mode", [ng.GRAPH_PLAIN, ng.GRAPH_BATCHED])
+def test_graph_c1(ctx, c1db, identity, mode):
+    _, _, d = c1db
+    say("graph c1 identity", identity, "mode", mode)
+    got, st = ctx.build_graph(d["packed"], d["word_off"], d["lengths"], d["rep"], 64, identity, mode)
+    want, tot = orc.build_graph(d["packed"], d["word_off"], d["lengths"], d["rep"], 64, identity)
+    assert st["total_pairs"] == tot
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("identity", [50, 90])
+def test_batched_pair_scores(ctx, c1db, identity):
+    """Every distinct pair's score from the length-binned kernels equals the oracle's."""
+    _, _, d = c1db
+    say("batched pair scores identity", identity)
+    db = ng.Database(ctx, d["packed"], d["word_off"], d["lengths"])
+    db.set_reps(d["rep"], 64)
+    db.graph(identity, 64, ng.GRAPH_BATCHED)
+    keys, sc = db.pair_scores()
+    jobs = np.zeros((len(keys), 3), dtype=np.uint32)
+    jobs[:, 0] = keys >> np.uint64(32)
+    jobs[:, 1] = keys & np.uint64(0xFFFFFFFF)
+    thr = float(np.float32(identity) / np.float32(100))
+    want = orc.align_jobs(d["packed"], d["word_off"], jobs, thr)
+    np.testing.assert_array_equal(sc, want[:, 2])
+
+
+@pytest.mark.parametrize("identity", [30, 50, 80, 95])
+def test_long_pairs(ctx, longdb, identity):
+    """Pairs routed to the wavefront-per-pair kernels (K = 1..8) and the plain bin."""
+    _, _, d = longdb
+    say("long pairs identity", identity)
+    db = ng.Database(ctx, d["packed"], d["word_off"], d["lengths"])
+    db.set_reps(d["rep"], 64)
+    db.graph(identity, 64, ng.GRAPH_BATCHED)
+    keys, sc = db.pair_scores()
+    jobs = np.zeros((len(keys), 3), dtype=np.uint32)
+    jobs[:, 0] = keys >> np.uint64(32)
+    jobs[:, 1] = keys & np.uint64(0xFFFFFFFF)
+    thr = float(np.float32(identity) / np.float32(100))
+    want = orc.align_jobs(d["packed"], d["word_off"], jobs, thr)
+    np.testing.assert_array_equal(sc, want[:, 2])
+    got, _ = ctx.build_graph(d["packed"], d["word_off"], d["lengths"], d["rep"], 64, identity,
+                             ng.GRAPH_BATCHED)
+    ref, _ = orc.build_graph(d["packed"], d["word_off"], d["lengths"], d["rep"], 64, identity)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_all_pairs_long(ctx, longdb):
+    """All ordered pairs of the long set through the batched kernels: every window bin."""
+    _, _, d = longdb
+    n = d["n"]
+    # rep rows that chain every sequence to each of the previous ones, band by band
+    rows = []
+    for b in range(64):
+        r = np.arange(n, dtype=np.int64) - 1 - b
+        r[r < 0] = np.arange(n)[r < 0]
+        rows.append(r.astype(np.uint32))
+    rep = np.stack(rows)
+    say("all pairs long", n)
+    for identity in (20, 60, 90):
+        db = ng.Database(ctx, d["packed"], d["word_off"], d["lengths"])
+        db.set_reps(rep, 64)
+        db.graph(identity, 64, ng.GRAPH_BATCHED)
+        keys, sc = db.pair_scores()
+        jobs = np.zeros((len(keys), 3), dtype=np.uint32)
+        jobs[:, 0] = keys >> np.uint64(32)
+        jobs[:, 1] = keys & np.uint64(0xFFFFFFFF)
+        thr = float(np.float32(identity) / np.float32(100))
+        want = orc.align_jobs(d["packed"], d["word_off"], jobs, thr)
+        np.testing.assert_array_equal(sc, want[:, 2])
+
+
+def test_makedb_bytes(ctx, c1db, tmp_path):
+    fa, db, _ = c1db
+    out = str(tmp_path / "mine.bin")
+    say("makedb c1")
+    ctx.makedb(fa, out)
+    assert open(out, "rb").read() == open(db, "rb").read()
+
+
+@pytest.mark.parametrize("identity", [50, 90])
+def test_cluster_result_file(ctx, c1db, tmp_path, identity):
+    fa, db, d = c1db
+    res = str(tmp_path / "r.txt")
+    say("cluster c1 identity", identity)
+    ctx.cluster(db, res, identity)
+    edges, _ = orc.build_graph(d["packed"], d["word_off"], d["lengths"], d["rep"], 64, identity)
+    cl = orc.leiden(edges, d["n"], orc.resolution(identity))
+    want = str(tmp_path / "w.txt")
+    orc.save_result(cl, db, want)
+    assert open(res, "rb").read() == open(want, "rb").read()
+
+
+@pytest.mark.parametrize("identity", [50, 90])
+def test_fast_mode(ctx, c1db, identity):
+    _, _, d = c1db
+    say("fast c1 identity", identity)
+    db = ng.Database(ctx, d["packed"], d["word_off"], d["lengths"])
+    db.set_reps(d["rep"], 64)
+    got, tot = db.fast(identity, 64)
+    want, wtot = orc.fast_cluster(d["packed"], d["word_off"], d["lengths"], d["rep"], 64, identity)
+    assert tot == wtot
+    np.testing.assert_array_equal(got, want)
+
+
+def test_db_pipeline_matches_oracle_db(ctx, c1db):
+    """Resident DB: sketch + bucket on the device give the rep rows of the oracle DB."""
+    _, _, d = c1db
+    say("db sketch+bucket c1")
+    db = ng.Database(ctx, d["packed"], d["word_off"], d["lengths"])
+    db.sketch()
+    db.bucket()
+    np.testing.assert_array_equal(db.reps(), d["rep"])
+    np.testing.assert_array_equal(db.sigs(), orc.sketch_all(d["packed"], d["word_off"]))
+
+
+def test_graph_replay(ctx, c1db):
+    """The captured batched round replays to the same edges, interleaved with eager work."""
+    _, _, d = c1db
+    say("graph replay")
+    db = ng.Database(ctx, d["packed"], d["word_off"], d["lengths"])
+    db.set_reps(d["rep"], 64)
+    want, _ = orc.build_graph(d["packed"], d["word_off"], d["lengths"], d["rep"], 64, 70)
+    for k in range(4):
+        db.graph(70, 64, ng.GRAPH_BATCHED)
+        np.testing.assert_array_equal(db.edges(), want)
+        db.graph(70, 64, ng.GRAPH_PLAIN)  # eager work between replays
+        np.testing.assert_array_equal(db.edges(), want)
