@@ -83,6 +83,16 @@ void ngia_close(ngia_ctx* ctx);
 const char* ngia_last_error(const ngia_ctx* ctx);
 int ngia_device_query(ngia_ctx* ctx, ngia_device_info* info);
 int ngia_sync(ngia_ctx* ctx);
+/* Device time (HIP events on the context stream) of the most recent stages.
+ * No reference counterpart: replaces the reference's wall-clock phase timers
+ * (buildGraph.hpp:272, 352-361). */
+typedef struct {
+  double sketch_ms;   /* k_sketch of the last ngia_db_sketch */
+  double bucket_ms;   /* transpose + radix sort + predecessor of the last ngia_db_bucket */
+  double graph_ms;    /* candidates .. raw edges of the last ngia_db_graph */
+  double finalize_ms; /* sort + unique of the last ngia_db_graph */
+} ngia_timing;
+int ngia_get_timing(ngia_ctx* ctx, ngia_timing* out);
 void ngia_free(void* p);
 
 /* ------------------------------------------- the three reference seams */

@@ -88,6 +88,14 @@ class Context:
                 "warp_size": d.warp_size, "hbm_total": d.hbm_total, "hbm_free": d.hbm_free,
                 "name": d.name.decode(), "arch": d.arch.decode()}
 
+    def timing(self):
+        t = _lib.Timing()
+        _check(_lib.lib().ngia_get_timing(self.h, C.byref(t)), self.h)
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def sync(self):
+        _check(_lib.lib().ngia_sync(self.h), self.h)
+
     # ---- the three reference seams, host pointers
     def sketch(self, packed, word_off):
         packed, word_off = _u32(packed), _u64(word_off)

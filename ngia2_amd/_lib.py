@@ -44,6 +44,11 @@ class GraphStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Timing(C.Structure):
+    _fields_ = [("sketch_ms", C.c_double), ("bucket_ms", C.c_double), ("graph_ms", C.c_double),
+                ("finalize_ms", C.c_double)]
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [
         ("device", C.c_int32),
@@ -105,6 +110,7 @@ PROTOTYPES = {
     "ngia_last_error": (C.c_char_p, [P]),
     "ngia_device_query": (C.c_int, [P, C.POINTER(DeviceInfo)]),
     "ngia_sync": (C.c_int, [P]),
+    "ngia_get_timing": (C.c_int, [P, C.POINTER(Timing)]),
     "ngia_free": (None, [P]),
     "ngia_sketch": (C.c_int, [P, P, U64, P, U32, P]),
     "ngia_bucket": (C.c_int, [P, P, U32, P]),

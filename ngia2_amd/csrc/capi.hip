@@ -26,6 +26,8 @@ using namespace ngia;
 
 namespace ngia {
 
+constexpr int kSideStreams = 4;
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -75,10 +77,21 @@ struct ngia_ctx {
   hipDeviceProp_t prop{};
   std::string err;
   hipEvent_t ev[8] = {};
+  // side streams (default priority) for the concurrent alignment bins
+  hipStream_t side[kSideStreams] = {};
+  hipEvent_t fork = nullptr, join[kSideStreams] = {};
   uint32_t wave_kmax = 8;  // largest k_align_wave<K> usable on this device
+  // NGIA_BUCKET_WIDE=1 selects the 38-bit-key bucketing that n >= 2^26 uses
+  bool bucket_wide = false;
+  ngia_timing timing{};    // device time of the last sketch / bucket / graph
   ~ngia_ctx() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
+    for (auto& e : join)
+      if (e) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    for (auto& q : side)
+      if (q) (void)hipStreamDestroy(q);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -298,7 +311,10 @@ void graph_plain(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
   finalize(ctx, db, ecount);
   NGIA_HIP(hipEventRecord(ctx->ev[5], s));
   NGIA_HIP(hipEventSynchronize(ctx->ev[5]));
-  ms_edges += event_ms(ctx->ev[4], ctx->ev[5]);
+  const float ms_fin = event_ms(ctx->ev[4], ctx->ev[5]);
+  ctx->timing.graph_ms = ms_cand + ms_align + ms_edges;
+  ctx->timing.finalize_ms = ms_fin;
+  ms_edges += ms_fin;
   if (st) {
     st->ms_edges = ms_edges;
     st->n_edges = db->n_edges;
@@ -323,18 +339,28 @@ void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ban
             dc->bin_off, dc->cursor, db->list.as<uint32_t>(), &dc->tiles, db->score.as<uint32_t>(),
             ctx->wave_kmax, grid, s);
   const AlignArgs aa = align_args(db, thr);
-  // widest windows first: their pairs take longest
+  // The bins run concurrently: forked onto the side streams (round-robin,
+  // widest windows first since their pairs take longest) and joined back, so
+  // a small bin's tail does not leave the GPU idle.
+  NGIA_HIP(hipEventRecord(ctx->fork, s));
+  for (int k = 0; k < kSideStreams; k++) NGIA_HIP(hipStreamWaitEvent(ctx->side[k], ctx->fork, 0));
+  int slot = 0;
   for (int k = 5; k >= 0; k--) {
     if ((1u << k) > ctx->wave_kmax) continue;
     const int bin = kBinWave0 + k;
     launch_align_wave(1 << k, aa, db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[bin],
                       &dc->bin_cnt[bin], &dc->work[bin], db->score.as<uint32_t>(),
-                      persistent_blocks(ctx, wave_lds_bytes(1 << k)), s);
+                      persistent_blocks(ctx, wave_lds_bytes(1 << k)),
+                      ctx->side[slot++ % kSideStreams]);
   }
   for (int W = kWinMax; W >= 2; W--) {
     launch_align_win(W, aa, db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[W],
                      &dc->bin_cnt[W], &dc->work[W], db->score.as<uint32_t>(),
-                     persistent_blocks(ctx, win_lds_bytes(W)), s);
+                     persistent_blocks(ctx, win_lds_bytes(W)), ctx->side[slot++ % kSideStreams]);
+  }
+  for (int k = 0; k < kSideStreams; k++) {
+    NGIA_HIP(hipEventRecord(ctx->join[k], ctx->side[k]));
+    NGIA_HIP(hipStreamWaitEvent(s, ctx->join[k], 0));
   }
   // windows beyond 64*32 words: the plain kernel over job triplets
   plain_bin_jobs(db->pairs.as<uint2>(), db->list.as<uint32_t>(), dc->bin_off, dc->bin_cnt,
@@ -403,6 +429,8 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   finalize(ctx, db, hc.ecount);
   NGIA_HIP(hipEventRecord(ctx->ev[5], s));
   NGIA_HIP(hipEventSynchronize(ctx->ev[5]));
+  ctx->timing.graph_ms = event_ms(ctx->ev[0], ctx->ev[1]);
+  ctx->timing.finalize_ms = event_ms(ctx->ev[4], ctx->ev[5]);
   if (st) {
     st->total_pairs = hc.total_cand;
     st->distinct_pairs = npairs;
@@ -432,8 +460,13 @@ extern "C" int ngia_open(int device, ngia_ctx** out) {
     NGIA_HIP(hipSetDevice(device));
     NGIA_HIP(hipGetDeviceProperties(&ctx->prop, device));
     ctx->wave_kmax = (uint32_t)align_kernels_setup(ctx->prop.sharedMemPerBlock);
+    const char* bw = std::getenv("NGIA_BUCKET_WIDE");
+    ctx->bucket_wide = bw && bw[0] == '1';
     NGIA_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (auto& e : ctx->ev) NGIA_HIP(hipEventCreate(&e));
+    for (auto& q : ctx->side) NGIA_HIP(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+    NGIA_HIP(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
+    for (auto& e : ctx->join) NGIA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return NGIA_OK;
   });
   if (rc != NGIA_OK) return rc;
@@ -468,6 +501,12 @@ extern "C" int ngia_device_query(ngia_ctx* ctx, ngia_device_info* info) {
     std::snprintf(info->arch, sizeof(info->arch), "%s", ctx->prop.gcnArchName);
     return NGIA_OK;
   });
+}
+
+extern "C" int ngia_get_timing(ngia_ctx* ctx, ngia_timing* out) {
+  if (!ctx || !out) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  *out = ctx->timing;
+  return NGIA_OK;
 }
 
 extern "C" int ngia_sync(ngia_ctx* ctx) {
@@ -505,9 +544,12 @@ extern "C" int ngia_db_sketch(ngia_ctx* ctx, ngia_db* db) {
   if (!ctx || !db) return fail(ctx, NGIA_ERR_ARG, "null argument");
   return guarded(ctx, [&] {
     db->sig.ensure(256ull * std::max<uint32_t>(db->n, 1));
+    NGIA_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
     launch_sketch(db->codes.as<uint8_t>(), db->code_off.as<uint64_t>(), db->lengths.as<uint32_t>(),
                   db->n, db->sig.as<uint32_t>(), ctx->stream);
+    NGIA_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
     NGIA_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->timing.sketch_ms = event_ms(ctx->ev[6], ctx->ev[7]);
     db->have_sig = true;
     return NGIA_OK;
   });
@@ -518,10 +560,13 @@ extern "C" int ngia_db_bucket(ngia_ctx* ctx, ngia_db* db) {
   if (!db->have_sig) return fail(ctx, NGIA_ERR_STATE, "bucket before sketch");
   return guarded(ctx, [&] {
     db->rep.ensure(256ull * std::max<uint32_t>(db->n, 1));
-    db->bucket_scratch.ensure(bucket_scratch_bytes(db->n));
+    db->bucket_scratch.ensure(bucket_scratch_bytes(db->n, ctx->bucket_wide));
+    NGIA_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
     launch_bucket(db->sig.as<uint32_t>(), db->n, db->rep.as<uint32_t>(), db->bucket_scratch.p,
-                  ctx->stream);
+                  ctx->stream, ctx->bucket_wide);
+    NGIA_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
     NGIA_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->timing.bucket_ms = event_ms(ctx->ev[6], ctx->ev[7]);
     db->rep_bands = NGIA_BANDS;
     if (db->gexec) {
       (void)hipGraphExecDestroy(db->gexec);
@@ -681,9 +726,10 @@ extern "C" int ngia_bucket(ngia_ctx* ctx, const uint32_t* sig, uint32_t n, uint3
     DevBuf dsig, drep, scratch;
     dsig.ensure(256ull * n);
     drep.ensure(256ull * n);
-    scratch.ensure(bucket_scratch_bytes(n));
+    scratch.ensure(bucket_scratch_bytes(n, ctx->bucket_wide));
     h2d(dsig.p, sig, 256ull * n, ctx->stream);
-    launch_bucket(dsig.as<uint32_t>(), n, drep.as<uint32_t>(), scratch.p, ctx->stream);
+    launch_bucket(dsig.as<uint32_t>(), n, drep.as<uint32_t>(), scratch.p, ctx->stream,
+                  ctx->bucket_wide);
     d2h(rep, drep.p, 256ull * n, ctx->stream);
     NGIA_HIP(hipStreamSynchronize(ctx->stream));
     return NGIA_OK;

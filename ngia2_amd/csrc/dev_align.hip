@@ -163,6 +163,85 @@ void launch_align_plain(const AlignArgs& a, uint32_t* jobs, uint32_t count, uint
 // ------------------------------------------------ short window, thread/pair
 // Pair list entry: pairs[p] = (rep, tar); geometry computed by the binning
 // pass so that a + c == W for every pair of the list.
+//
+// Block cb of a W-block round uses register and Peq slot (cb + j) mod W for
+// relative word j; the round is instantiated W times (Q = cb mod W) so that
+// every index is a compile-time constant.
+template <int W>
+struct WinPair {
+  uint32_t* rows;        // W row words (registers after inlining)
+  uint32_t lcs;          // zero bits of the words that left the window
+  const uint32_t* rr;    // rep record
+  const uint32_t* tc;    // target byte codes, 8 dwords per block
+  int32_t c, nw;
+  uint32_t nb;
+  uint32_t* lds;         // [W][16][64] Peq slots, then [8][64] staged codes
+  uint32_t lane, lane4;
+  uint4 nc0, nc1;        // codes of the next block, loaded one block ahead
+};
+
+template <int W, int Q>
+__device__ __forceinline__ bool win_block(WinPair<W>& s, uint32_t cb0) {
+  const uint32_t cb = cb0 + Q;
+  if (cb >= s.nb) return false;
+  uint32_t* const lds = s.lds;
+  uint32_t* const cst = lds + W * 1024;
+  const uint32_t lane = s.lane;
+  // This block's codes arrived during the previous block; fetch the next
+  // block's now (past the last block the read stays inside the buffer's
+  // 64-byte tail padding), and the planes of the word entering at the end.
+  const uint4 c0 = s.nc0, c1 = s.nc1;
+  s.nc0 = *(const uint4*)(s.tc + 8 * (cb + 1));
+  s.nc1 = *(const uint4*)(s.tc + 8 * (cb + 1) + 4);
+  const int32_t w = (int32_t)cb + s.c;
+  uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+  if (w < s.nw) {
+    const uint32_t* pl = s.rr + 1 + 4 * w;
+    p0 = pl[0]; p1 = pl[1]; p2 = pl[2]; p3 = pl[3];
+  }
+  // Stage this block's 32 target codes (4 per dword) in LDS.
+  cst[0 * 64 + lane] = c0.x; cst[1 * 64 + lane] = c0.y;
+  cst[2 * 64 + lane] = c0.z; cst[3 * 64 + lane] = c0.w;
+  cst[4 * 64 + lane] = c1.x; cst[5 * 64 + lane] = c1.y;
+  cst[6 * 64 + lane] = c1.z; cst[7 * 64 + lane] = c1.w;
+  for (int g = 0; g < 8; g++) {
+    const uint32_t cw = cst[g * 64 + lane];
+    #pragma unroll
+    for (int mm = 0; mm < 4; mm++) {
+      // byte address of Peq[code][lane] = code*256 + lane*4: v_perm takes
+      // byte 0 from lane4 (selector 0) and byte 1 from code byte mm of cw
+      // (selector 4+mm); bytes 2,3 are zero (selector 0x0C).
+      const uint32_t addr =
+          __builtin_amdgcn_perm(cw, s.lane4, 0x0C0C0000u | ((4u + (uint32_t)mm) << 8));
+      const uint32_t* pq = (const uint32_t*)((const char*)lds + addr);
+      uint32_t cin = 0;
+      #pragma unroll
+      for (int j = 0; j < W; j++) {
+        constexpr int kW = W;
+        const int r = (Q + j) % kW;  // register and Peq slot of relative word j
+        s.rows[r] = lcs_step(s.rows[r], pq[r * 1024], cin, &cin);
+      }
+    }
+  }
+  // Slide: relative word 0 (register Q) leaves, word cb + c enters there.
+  s.lcs += 32u - __popc(s.rows[Q]);
+  s.rows[Q] = 0xFFFFFFFFu;
+  uint32_t m[16];
+  // Past the rep's last word the planes are 0, and so is every mask of a
+  // nonzero code (each needs a set plane bit or a clear one with another set).
+  peq_masks(p0, p1, p2, p3, m);
+  #pragma unroll
+  for (int k = 1; k < 16; k++) lds[Q * 1024 + k * 64 + lane] = m[k];
+  return true;
+}
+
+template <int W, int Q>
+__device__ __forceinline__ void win_round(WinPair<W>& s, uint32_t cb0) {
+  if constexpr (Q < W) {
+    if (win_block<W, Q>(s, cb0)) win_round<W, Q + 1>(s, cb0);
+  }
+}
+
 template <int W>
 __global__ __launch_bounds__(64) void k_align_win(AlignArgs A, const uint2* __restrict__ pairs,
                                                   const uint32_t* __restrict__ list,
@@ -215,55 +294,12 @@ __global__ __launch_bounds__(64) void k_align_win(AlignArgs A, const uint2* __re
       #pragma unroll
       for (int k = 1; k < 16; k++) lds[j * 1024 + k * 64 + lane] = m[k];
     }
-    uint32_t lcs = 0;
-    for (uint32_t cb0 = 0; cb0 < nb; cb0 += W) {
-      #pragma unroll
-      for (int q = 0; q < W; q++) {
-        const uint32_t cb = cb0 + q;
-        if (cb >= nb) break;
-        // Stage this block's 32 target codes (4 per word) in LDS.
-        const uint4 c0 = *(const uint4*)(tc + 8 * cb), c1 = *(const uint4*)(tc + 8 * cb + 4);
-        cst[0 * 64 + lane] = c0.x; cst[1 * 64 + lane] = c0.y;
-        cst[2 * 64 + lane] = c0.z; cst[3 * 64 + lane] = c0.w;
-        cst[4 * 64 + lane] = c1.x; cst[5 * 64 + lane] = c1.y;
-        cst[6 * 64 + lane] = c1.z; cst[7 * 64 + lane] = c1.w;
-        for (int g = 0; g < 8; g++) {
-          const uint32_t cw = cst[g * 64 + lane];
-          #pragma unroll
-          for (int mm = 0; mm < 4; mm++) {
-            // byte address of Peq[code][lane] = code*256 + lane*4: v_perm takes
-            // byte 0 from lane4 (selector 0) and byte 1 from code byte mm of cw
-            // (selector 4+mm); bytes 2,3 are zero (selector 0x0C).
-            const uint32_t addr =
-                __builtin_amdgcn_perm(cw, lane4, 0x0C0C0000u | ((4u + (uint32_t)mm) << 8));
-            const uint32_t* pq = (const uint32_t*)((const char*)lds + addr);
-            uint32_t cin = 0;
-            #pragma unroll
-            for (int j = 0; j < W; j++) {
-              const int r = (q + j) % W;  // register and Peq slot of relative word j
-              rows[r] = lcs_step(rows[r], pq[r * 1024], cin, &cin);
-            }
-          }
-        }
-        // Slide: relative word 0 (register q) leaves, word cb + c enters there.
-        lcs += 32u - __popc(rows[q]);
-        rows[q] = 0xFFFFFFFFu;
-        const int32_t w = (int32_t)cb + c;
-        uint32_t m[16];
-        if (w < nw) {
-          const uint32_t* pl = rr + 1 + 4 * w;
-          peq_masks(pl[0], pl[1], pl[2], pl[3], m);
-        } else {
-          #pragma unroll
-          for (int k = 0; k < 16; k++) m[k] = 0;
-        }
-        #pragma unroll
-        for (int k = 1; k < 16; k++) lds[q * 1024 + k * 64 + lane] = m[k];
-      }
-    }
+    WinPair<W> st{rows, 0u, rr, tc, c, nw, nb, lds, lane, lane4,
+                  *(const uint4*)tc, *(const uint4*)(tc + 4)};
+    for (uint32_t cb0 = 0; cb0 < nb; cb0 += W) win_round<W, 0>(st, cb0);
     #pragma unroll
-    for (int j = 0; j < W; j++) lcs += 32u - __popc(rows[j]);
-    score[p] = lcs_score(lcs, R, T, A.thr);
+    for (int j = 0; j < W; j++) st.lcs += 32u - __popc(rows[j]);
+    score[p] = lcs_score(st.lcs, R, T, A.thr);
   }
 }
 
@@ -406,6 +442,14 @@ void launch_align_win(int W, const AlignArgs& a, const uint2* pairs, const uint3
     NGIA_WIN_CASE(6)
     NGIA_WIN_CASE(7)
     NGIA_WIN_CASE(8)
+    NGIA_WIN_CASE(9)
+    NGIA_WIN_CASE(10)
+    NGIA_WIN_CASE(11)
+    NGIA_WIN_CASE(12)
+    NGIA_WIN_CASE(13)
+    NGIA_WIN_CASE(14)
+    NGIA_WIN_CASE(15)
+    NGIA_WIN_CASE(16)
     default:
       throw HipError{NGIA_ERR_ARG, "no window kernel for W=" + std::to_string(W)};
   }

@@ -60,10 +60,11 @@ __host__ __device__ inline uint32_t lcs_score(uint32_t lcs, uint32_t R, uint32_t
 // All take device pointers and run on `stream`.
 
 // Band-major bucketing of n x 64 signatures into 64 x n representative rows.
-// scratch holds bucket_scratch_bytes(n) bytes.
-size_t bucket_scratch_bytes(uint32_t n);
+// scratch holds bucket_scratch_bytes(n, force_wide) bytes; force_wide selects
+// the 38-bit-key form that large n needs anyway.
+size_t bucket_scratch_bytes(uint32_t n, bool force_wide = false);
 void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch,
-                   hipStream_t stream);
+                   hipStream_t stream, bool force_wide = false);
 
 // Byte-per-residue codes (code of residue j at byte code_off[i] + j, padding
 // to a multiple of 32 is code 0) built from the packed planes.
@@ -105,7 +106,7 @@ void launch_align_wave(int K, const AlignArgs& a, const uint2* pairs, const uint
 // Raise the dynamic-LDS limit of the kernels that need more than 64 KiB;
 // returns the largest K of k_align_wave<K> the device accepts.
 int align_kernels_setup(size_t lds_per_block);
-constexpr int kWinMax = 8;     // window widths 2..8 use k_align_win
+constexpr int kWinMax = 16;    // window widths 2..16 use k_align_win
 constexpr int kWaveKMax = 32;  // k_align_wave handles W <= 64*32
 
 }  // namespace ngia

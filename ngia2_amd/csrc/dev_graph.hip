@@ -160,15 +160,28 @@ void band_jobs(const uint32_t* rep, const uint32_t* len, uint32_t n, float thr,
   NGIA_HIP(hipGetLastError());
 }
 
+// Slot of this lane in an append-only list: one atomic per wavefront for the
+// lanes with `pred` set; returns ~0ull for lanes without it.
+__device__ inline unsigned long long wave_append(unsigned long long* counter, bool pred) {
+  const uint64_t m = __builtin_amdgcn_ballot_w64(pred);
+  if (!m) return ~0ull;
+  const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__builtin_popcountll(m));
+  base = __shfl(base, (int)leader);
+  const uint32_t rank = (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull));
+  return pred ? base + rank : ~0ull;
+}
+
 // Append the passing jobs of one band to the raw edge list.
 __global__ void k_jobs_to_edges(const uint32_t* __restrict__ jobs, uint32_t count,
                                 uint64_t* __restrict__ ekeys, uint32_t* __restrict__ escores,
                                 unsigned long long* __restrict__ ecount) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= count) return;
-  const uint32_t sc = jobs[3 * j + 2];
+  const uint32_t sc = j < count ? jobs[3 * j + 2] : 0u;
+  const unsigned long long at = wave_append(ecount, sc != 0);
   if (sc == 0) return;
-  const unsigned long long at = atomicAdd(ecount, 1ull);
   ekeys[at] = ((uint64_t)jobs[3 * j] << 32) | jobs[3 * j + 1];
   escores[at] = sc;
 }
@@ -197,11 +210,12 @@ void fast_apply(const uint32_t* jobs, uint32_t count, uint32_t* parent, hipStrea
 }
 
 // ------------------------------------------------------------- batched form
-// One wavefront per target i; lane b < bands reads rep[b][i].  The distinct
-// representatives passing the candidate predicate are kept in order of their
-// first band.  Count pass (pairs == nullptr) writes cnt[i] and adds the
-// candidate count with duplicates to *total; emit pass writes the pairs at
-// off[i].
+// A block takes 64 consecutive targets: the 64 x 64 tile rep[b][i0..i0+63]
+// is read row by row (coalesced) into LDS, then each wavefront handles 16
+// targets, lane b holding rep[b][i].  The distinct representatives passing
+// the candidate predicate are kept in order of their first band.  Count pass
+// (pairs == nullptr) writes cnt[i] and adds the candidate count with
+// duplicates to *total; emit pass writes the pairs at off[i].
 __global__ __launch_bounds__(256) void k_pair_scan(const uint32_t* __restrict__ rep,
                                                    const uint32_t* __restrict__ len, uint32_t n,
                                                    uint32_t bands, float thr,
@@ -209,46 +223,57 @@ __global__ __launch_bounds__(256) void k_pair_scan(const uint32_t* __restrict__ 
                                                    unsigned long long* __restrict__ total,
                                                    const uint64_t* __restrict__ off,
                                                    uint2* __restrict__ pairs) {
+  __shared__ uint32_t tile[64][65];
+  __shared__ unsigned long long btotal;
+  const uint32_t i0 = blockIdx.x * 64;
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t i = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (i >= n) return;
-  uint32_t r = 0xFFFFFFFFu;
-  bool valid = false;
-  if (lane < bands) {
-    r = rep[(uint64_t)lane * n + i];
-    valid = is_job(r, i, len, thr);
+  const uint32_t wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) btotal = 0;
+  for (uint32_t e = threadIdx.x; e < 64 * 64; e += 256) {
+    const uint32_t b = e >> 6, t = e & 63;
+    tile[b][t] = (b < bands && i0 + t < n) ? rep[(uint64_t)b * n + i0 + t] : 0u;
   }
-  const uint64_t vm = __builtin_amdgcn_ballot_w64(valid);
-  uint64_t mask = vm, keep = 0;
-  while (mask) {
-    const uint32_t b0 = (uint32_t)__builtin_ctzll(mask);
-    const uint32_t r0 = __builtin_amdgcn_readlane(r, b0);
-    keep |= 1ull << b0;
-    mask &= ~__builtin_amdgcn_ballot_w64(valid && r == r0);
+  __syncthreads();
+  uint32_t wtotal = 0;
+  for (uint32_t tt = 0; tt < 16; tt++) {
+    const uint32_t t = wave * 16 + tt;
+    const uint32_t i = i0 + t;
+    if (i >= n) break;
+    const uint32_t r = tile[lane][t];
+    const bool valid = lane < bands && is_job(r, i, len, thr);
+    const uint64_t vm = __builtin_amdgcn_ballot_w64(valid);
+    uint64_t mask = vm, keep = 0;
+    while (mask) {
+      const uint32_t b0 = (uint32_t)__builtin_ctzll(mask);
+      const uint32_t r0 = __builtin_amdgcn_readlane(r, b0);
+      keep |= 1ull << b0;
+      mask &= ~__builtin_amdgcn_ballot_w64(valid && r == r0);
+    }
+    if (!pairs) {
+      if (lane == 0) cnt[i] = (uint32_t)__builtin_popcountll(keep);
+      wtotal += (uint32_t)__builtin_popcountll(vm);
+    } else if ((keep >> lane) & 1ull) {
+      const uint32_t rank = (uint32_t)__builtin_popcountll(keep & ((1ull << lane) - 1ull));
+      pairs[off[i] + rank] = make_uint2(r, i);
+    }
   }
   if (!pairs) {
-    if (lane == 0) {
-      cnt[i] = (uint32_t)__builtin_popcountll(keep);
-      if (vm) atomicAdd(total, (unsigned long long)__builtin_popcountll(vm));
-    }
-    return;
-  }
-  if ((keep >> lane) & 1ull) {
-    const uint32_t rank = (uint32_t)__builtin_popcountll(keep & ((1ull << lane) - 1ull));
-    pairs[off[i] + rank] = make_uint2(r, i);
+    if (lane == 0 && wtotal) atomicAdd(&btotal, (unsigned long long)wtotal);
+    __syncthreads();
+    if (threadIdx.x == 0 && btotal) atomicAdd(total, btotal);
   }
 }
 
 void pair_count(const uint32_t* rep, const uint32_t* len, uint32_t n, uint32_t bands, float thr,
                 uint32_t* cnt, unsigned long long* total, hipStream_t s) {
-  hipLaunchKernelGGL(k_pair_scan, dim3((n + 3) / 4), dim3(256), 0, s, rep, len, n, bands, thr, cnt,
-                     total, (const uint64_t*)nullptr, (uint2*)nullptr);
+  hipLaunchKernelGGL(k_pair_scan, dim3((n + 63) / 64), dim3(256), 0, s, rep, len, n, bands, thr,
+                     cnt, total, (const uint64_t*)nullptr, (uint2*)nullptr);
   NGIA_HIP(hipGetLastError());
 }
 
 void pair_emit(const uint32_t* rep, const uint32_t* len, uint32_t n, uint32_t bands, float thr,
                const uint64_t* off, uint2* pairs, hipStream_t s) {
-  hipLaunchKernelGGL(k_pair_scan, dim3((n + 3) / 4), dim3(256), 0, s, rep, len, n, bands, thr,
+  hipLaunchKernelGGL(k_pair_scan, dim3((n + 63) / 64), dim3(256), 0, s, rep, len, n, bands, thr,
                      (uint32_t*)nullptr, (unsigned long long*)nullptr, off, pairs);
   NGIA_HIP(hipGetLastError());
 }
@@ -324,15 +349,36 @@ __global__ void k_bin_offsets(const uint32_t* __restrict__ bin_cnt, uint32_t* __
   }
 }
 
-__global__ void k_bin_scatter(const uint64_t* __restrict__ npairs, const uint32_t* __restrict__ bin_of,
-                              const uint32_t* __restrict__ bin_off, uint32_t* __restrict__ cursor,
-                              uint32_t* __restrict__ list) {
+// Scatter pair indices into their bin's segment.  A block takes tiles of
+// 4096 pairs: slots are counted in LDS, reserved with one global atomic per
+// (tile, bin), then written.
+__global__ __launch_bounds__(256) void k_bin_scatter(const uint64_t* __restrict__ npairs,
+                                                     const uint32_t* __restrict__ bin_of,
+                                                     const uint32_t* __restrict__ bin_off,
+                                                     uint32_t* __restrict__ cursor,
+                                                     uint32_t* __restrict__ list) {
+  __shared__ uint32_t lcnt[kBins], lbase[kBins];
   const uint64_t P = *npairs;
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
-       p += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t b = bin_of[p];
-    const uint32_t at = atomicAdd(&cursor[b], 1u);
-    list[bin_off[b] + at] = (uint32_t)p;
+  for (uint64_t base = (uint64_t)blockIdx.x * 4096; base < P; base += (uint64_t)gridDim.x * 4096) {
+    if (threadIdx.x < kBins) lcnt[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t b[16], pos[16];
+    #pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const uint64_t p = base + (uint64_t)k * 256 + threadIdx.x;
+      b[k] = p < P ? bin_of[p] : 0u;
+      pos[k] = p < P ? atomicAdd(&lcnt[b[k]], 1u) : 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x < kBins)
+      lbase[threadIdx.x] = lcnt[threadIdx.x] ? atomicAdd(&cursor[threadIdx.x], lcnt[threadIdx.x]) : 0u;
+    __syncthreads();
+    #pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const uint64_t p = base + (uint64_t)k * 256 + threadIdx.x;
+      if (p < P) list[bin_off[b[k]] + lbase[b[k]] + pos[k]] = (uint32_t)p;
+    }
+    __syncthreads();
   }
 }
 
@@ -388,9 +434,9 @@ __global__ void k_pairs_to_edges(const uint2* __restrict__ pairs, const uint64_t
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
        p += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t sc = score[p];
+    const unsigned long long at = wave_append(ecount, sc != 0);
     if (sc == 0) continue;
     const uint2 pr = pairs[p];
-    const unsigned long long at = atomicAdd(ecount, 1ull);
     ekeys[at] = ((uint64_t)pr.x << 32) | pr.y;
     escores[at] = sc;
   }

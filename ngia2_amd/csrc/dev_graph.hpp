@@ -8,7 +8,7 @@ namespace ngia {
 // Pair bins: 0 = scored 0 without alignment, 2..kWinMax = k_align_win<W>,
 // kBinWave0 + log2(K) = k_align_wave<K>, kBinPlain = the plain kernel.
 constexpr int kBinZero = 0;
-constexpr int kBinWave0 = kWinMax + 1;  // 9..14 for K = 1..32
+constexpr int kBinWave0 = kWinMax + 1;  // kWinMax+1 .. +6 for K = 1..32
 constexpr int kBinPlain = kBinWave0 + 6;
 constexpr int kBins = kBinPlain + 1;
 

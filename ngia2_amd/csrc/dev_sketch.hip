@@ -81,11 +81,22 @@ void launch_sketch(const uint8_t* codes, const uint64_t* code_off, const uint32_
 }
 
 // ------------------------------------------------------------- bucketing
-// Keys (band << 32 | sig) in band-major order with values i ascending inside
-// each band; a stable radix sort on the low 38 bits then yields, per band,
-// the (sig, i)-lexicographic order the reference's LSD radix produces.
+// Per band the reference stable-sorts (sig, i) by sig (makedb.hpp:320-339)
+// and points every element at its predecessor in the equal-signature run
+// (makedb.hpp:340-347).  Here all 64 bands are sorted at once by a stable
+// radix sort on the 32-bit signature alone, the value carrying (band, i)
+// with the band in the top bits: the input is band-major with i ascending,
+// so after the stable sort every equal-signature run lists its elements by
+// band, then by i -- each band's run is a contiguous, i-ascending slice, and
+// an element's predecessor is the previous element when that one has the
+// same signature and the same band.  Rows start as the identity; only the
+// non-head elements are scattered.  Values need n < 2^26; larger n uses the
+// (band << 32 | sig) 38-bit key instead.
+constexpr uint32_t kIdxBits = 26;
+constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
+
 __global__ __launch_bounds__(256) void k_bucket_keys(const uint32_t* __restrict__ sig, uint32_t n,
-                                                     uint64_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ keys,
                                                      uint32_t* __restrict__ vals) {
   // 64 sequences x 64 bands tile through LDS: coalesced reads of the
   // sequence-major signatures, coalesced writes of band-major keys.
@@ -101,39 +112,70 @@ __global__ __launch_bounds__(256) void k_bucket_keys(const uint32_t* __restrict_
     const uint32_t b = e >> 6, si = e & 63;
     const uint32_t i = i0 + si;
     if (i < n) {
-      keys[(uint64_t)b * n + i] = ((uint64_t)b << 32) | tile[si][b];
-      vals[(uint64_t)b * n + i] = i;
+      keys[(uint64_t)b * n + i] = tile[si][b];
+      vals[(uint64_t)b * n + i] = (b << kIdxBits) | i;
     }
   }
 }
 
-__global__ __launch_bounds__(256) void k_bucket_rep(const uint64_t* __restrict__ keys,
+__global__ __launch_bounds__(256) void k_rep_identity(uint32_t n, uint32_t* __restrict__ rep) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;  // blockIdx.y = band
+  if (i < n) rep[(uint64_t)blockIdx.y * n + i] = i;
+}
+
+__global__ __launch_bounds__(256) void k_bucket_rep(const uint32_t* __restrict__ keys,
                                                     const uint32_t* __restrict__ vals,
                                                     uint64_t total, uint32_t n,
                                                     uint32_t* __restrict__ rep) {
   const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (p >= total) return;
+  if (p == 0 || p >= total) return;
+  const uint32_t v = vals[p], pv = vals[p - 1];
+  if (keys[p] == keys[p - 1] && (v >> kIdxBits) == (pv >> kIdxBits))
+    rep[(uint64_t)(v >> kIdxBits) * n + (v & kIdxMask)] = pv & kIdxMask;
+}
+
+// Wide form for n >= 2^26: keys (band << 32 | sig), values i.
+__global__ __launch_bounds__(256) void k_bucket_keys_wide(const uint32_t* __restrict__ sig,
+                                                          uint32_t n, uint64_t* __restrict__ keys,
+                                                          uint32_t* __restrict__ vals) {
+  const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (uint64_t)n * 64) return;
+  const uint32_t b = (uint32_t)(e / n), i = (uint32_t)(e % n);
+  keys[e] = ((uint64_t)b << 32) | sig[(uint64_t)i * 64 + b];
+  vals[e] = i;
+}
+
+__global__ __launch_bounds__(256) void k_bucket_rep_wide(const uint64_t* __restrict__ keys,
+                                                         const uint32_t* __restrict__ vals,
+                                                         uint64_t total, uint32_t n,
+                                                         uint32_t* __restrict__ rep) {
+  const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p == 0 || p >= total) return;
   const uint64_t k = keys[p];
-  const uint32_t i = vals[p];
-  const uint32_t b = (uint32_t)(k >> 32);
-  const bool same = p > 0 && keys[p - 1] == k;
-  rep[(uint64_t)b * n + i] = same ? vals[p - 1] : i;
+  if (keys[p - 1] == k) rep[(k >> 32) * n + vals[p]] = vals[p - 1];
 }
 
 namespace {
 struct BucketLayout {
+  bool wide;
   size_t keys_in, keys_out, vals_in, vals_out, temp, temp_bytes, total;
 };
-BucketLayout bucket_layout(uint32_t n) {
+BucketLayout bucket_layout(uint32_t n, bool force_wide) {
   const uint64_t m = (uint64_t)n * 64;
-  size_t temp_bytes = 0;
-  (void)rocprim::radix_sort_pairs(nullptr, temp_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
-                                  (uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)m, 0, 38);
-  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   BucketLayout L;
+  L.wide = force_wide || n > kIdxMask;
+  const size_t kb = L.wide ? 8 : 4;
+  size_t temp_bytes = 0;
+  if (L.wide)
+    (void)rocprim::radix_sort_pairs(nullptr, temp_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                    (uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)m, 0, 38);
+  else
+    (void)rocprim::radix_sort_pairs(nullptr, temp_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                    (uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)m, 0, 32);
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   L.keys_in = 0;
-  L.keys_out = L.keys_in + al(8 * m);
-  L.vals_in = L.keys_out + al(8 * m);
+  L.keys_out = L.keys_in + al(kb * m);
+  L.vals_in = L.keys_out + al(kb * m);
   L.vals_out = L.vals_in + al(4 * m);
   L.temp = L.vals_out + al(4 * m);
   L.temp_bytes = temp_bytes;
@@ -142,26 +184,42 @@ BucketLayout bucket_layout(uint32_t n) {
 }
 }  // namespace
 
-size_t bucket_scratch_bytes(uint32_t n) { return bucket_layout(n).total; }
+size_t bucket_scratch_bytes(uint32_t n, bool force_wide) {
+  return bucket_layout(n, force_wide).total;
+}
 
 void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch,
-                   hipStream_t stream) {
+                   hipStream_t stream, bool force_wide) {
   if (n == 0) return;
-  const BucketLayout L = bucket_layout(n);
+  const BucketLayout L = bucket_layout(n, force_wide);
   char* s = (char*)scratch;
-  uint64_t* keys_in = (uint64_t*)(s + L.keys_in);
-  uint64_t* keys_out = (uint64_t*)(s + L.keys_out);
+  const uint64_t m = (uint64_t)n * 64;
   uint32_t* vals_in = (uint32_t*)(s + L.vals_in);
   uint32_t* vals_out = (uint32_t*)(s + L.vals_out);
-  const uint64_t m = (uint64_t)n * 64;
-  hipLaunchKernelGGL(k_bucket_keys, dim3((n + 63) / 64), dim3(256), 0, stream, sig, n, keys_in,
-                     vals_in);
-  NGIA_HIP(hipGetLastError());
   size_t tb = L.temp_bytes;
-  NGIA_HIP(rocprim::radix_sort_pairs(s + L.temp, tb, keys_in, keys_out, vals_in, vals_out,
-                                     (size_t)m, 0, 38, stream));
-  hipLaunchKernelGGL(k_bucket_rep, dim3(grid_for(m, 256)), dim3(256), 0, stream, keys_out,
-                     vals_out, m, n, rep);
+  hipLaunchKernelGGL(k_rep_identity, dim3(grid_for(n, 256), 64), dim3(256), 0, stream, n, rep);
+  NGIA_HIP(hipGetLastError());
+  if (!L.wide) {
+    uint32_t* keys_in = (uint32_t*)(s + L.keys_in);
+    uint32_t* keys_out = (uint32_t*)(s + L.keys_out);
+    hipLaunchKernelGGL(k_bucket_keys, dim3((n + 63) / 64), dim3(256), 0, stream, sig, n, keys_in,
+                       vals_in);
+    NGIA_HIP(hipGetLastError());
+    NGIA_HIP(rocprim::radix_sort_pairs(s + L.temp, tb, keys_in, keys_out, vals_in, vals_out,
+                                       (size_t)m, 0, 32, stream));
+    hipLaunchKernelGGL(k_bucket_rep, dim3(grid_for(m, 256)), dim3(256), 0, stream, keys_out,
+                       vals_out, m, n, rep);
+  } else {
+    uint64_t* keys_in = (uint64_t*)(s + L.keys_in);
+    uint64_t* keys_out = (uint64_t*)(s + L.keys_out);
+    hipLaunchKernelGGL(k_bucket_keys_wide, dim3(grid_for(m, 256)), dim3(256), 0, stream, sig, n,
+                       keys_in, vals_in);
+    NGIA_HIP(hipGetLastError());
+    NGIA_HIP(rocprim::radix_sort_pairs(s + L.temp, tb, keys_in, keys_out, vals_in, vals_out,
+                                       (size_t)m, 0, 38, stream));
+    hipLaunchKernelGGL(k_bucket_rep_wide, dim3(grid_for(m, 256)), dim3(256), 0, stream, keys_out,
+                       vals_out, m, n, rep);
+  }
   NGIA_HIP(hipGetLastError());
 }
 

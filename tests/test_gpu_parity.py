@@ -79,6 +79,20 @@ def test_bucket_c1(ctx, c1db):
     np.testing.assert_array_equal(ctx.bucket(sig), d["rep"])
 
 
+def test_bucket_wide_keys(c1db):
+    """The 38-bit (band, sig) key form that n >= 2^26 needs gives the same rows."""
+    _, _, d = c1db
+    sig = orc.sketch_all(d["packed"], d["word_off"])
+    os.environ["NGIA_BUCKET_WIDE"] = "1"
+    try:
+        say("bucket wide keys", d["n"])
+        with ng.Context(0) as c2:
+            got = c2.bucket(sig)
+    finally:
+        del os.environ["NGIA_BUCKET_WIDE"]
+    np.testing.assert_array_equal(got, d["rep"])
+
+
 def test_bucket_collisions(ctx):
     # heavy signature collisions: few distinct values per band
     rng = np.random.default_rng(5)
