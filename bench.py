@@ -3,16 +3,25 @@
 
 One step = one pass of the hot path over the resident database: minhash
 sketch of every sequence, band bucketing (64 bands), then the graph build at
-the given identity (candidates, alignment of every candidate pair, sorted
-unique edge list).  The database (packed records, offsets, lengths) is
-uploaded before the timed region.  Workload: config 2 of the benchmark suite,
-1,000,000 synthetic proteins in 50,000 families, at -i 90.
+-i 90 (candidates, alignment of every candidate pair, sorted unique edge
+list).  The database (packed records, offsets, lengths) is uploaded before
+the timed region.
+
+Workload: 1,000,000 synthetic proteins in 50,000 families with heavy-tailed
+lengths (log-normal body, median 300, plus a Pareto tail with alpha 1.5 up to
+35,000 residues), 0-40% divergence, at -i 90 -- the set on which the batched
+form is compared with the plain one-launch-per-band form.  The 1M-protein
+set with a log-normal length body only (median 350, at most 5,000) is
+measured the same way and reported beside it.
 
 Prints one JSON line.  `value` is sequences through the whole hot path per
-second in the batched form; the plain one-launch-per-band form is measured
-on the same GPU and reported beside it.  `cpu_baseline` times the oracle (the
-reference algorithm's CPU restatement, OpenMP) on a bounded sample of the
-same workload on this host.
+second in the batched form (captured hipGraph).  The plain form is measured
+on the same GPU and reported beside it.  A second pass of the same steps in
+the uncaptured batched form times every alignment kernel on its own stream
+with HIP events; from it come the alignment rooflines (reference 32x32 tiles
+x 128 int32 ops per tile / kernel time, against 64 lanes x CUs x clock).
+`cpu_baseline` times the oracle (the reference algorithm's CPU restatement,
+OpenMP over all host threads) on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -23,7 +32,12 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_HBM_GBS = 8000.0          # MI355X HBM3E, public figure
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, public figure
+OPS_PER_TILE = 128     # 32 rep-word x 32 target-char updates, 4 int32 ops each
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
 
 def valu_peak_tops(info):
@@ -42,11 +56,107 @@ def sketch_work(lengths):
     return ops, nbytes
 
 
-def cpu_baseline(config, scale, seed, identity):
-    """The oracle's sketch + bucket + graph over a config sample, all host threads."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import ngia2_amd as ng
-    import oracle_lib as orc
+def kernel_name(b):
+    if b["kind"] == "window":
+        return "k_align_win<%d>" % b["width"]
+    if b["kind"] == "wave":
+        return "k_align_wave<%d>" % b["width"]
+    return "k_align_plain"
+
+
+def load_traffic():
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    scaled by a plain copy of known size), keyed by kernel and workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def measure(ng, np, ctx, info, d, identity, steps, warmup, plain_steps, label):
+    db = ng.Database(ctx, d.packed, d.word_off, d.lengths)
+
+    def step(mode):
+        db.sketch()
+        t_s = ctx.timing()["sketch_ms"]
+        db.bucket()
+        t_b = ctx.timing()["bucket_ms"]
+        st = db.graph(identity, 64, mode)
+        tm = ctx.timing()
+        r = {"sketch_ms": t_s, "bucket_ms": t_b, "graph_ms": tm["graph_ms"],
+             "finalize_ms": tm["finalize_ms"], "align_ms": tm["align_ms"], "stats": st}
+        if mode == ng.GRAPH_BATCHED_EAGER:
+            r["bins"] = db.bin_stats()
+        return r
+
+    def run(mode, k, w):
+        for _ in range(w):
+            step(mode)
+        ctx.sync()
+        rec = []
+        t0 = time.perf_counter()
+        for _ in range(k):
+            rec.append(step(mode))
+        ctx.sync()
+        return time.perf_counter() - t0, rec
+
+    out = {}
+    dt, rec = run(ng.GRAPH_BATCHED, steps, warmup)
+    log("%s batched: %.2f ms/step" % (label, 1e3 * dt / steps))
+    mean = lambda rs, k: float(np.mean([r[k] for r in rs]))
+    out["dt"], out["steps"] = dt, steps
+    out["value"] = d.n * steps / dt
+    out["ms_per_step"] = 1e3 * dt / steps
+    out["stats"] = rec[-1]["stats"]
+    for k in ("sketch_ms", "bucket_ms", "graph_ms", "finalize_ms"):
+        out[k] = mean(rec, k)
+
+    # the same steps, uncaptured, with every alignment kernel timed on its stream
+    edt, erec = run(ng.GRAPH_BATCHED_EAGER, steps, 1)
+    log("%s eager batched: %.2f ms/step" % (label, 1e3 * edt / steps))
+    out["eager_ms_per_step"] = 1e3 * edt / steps
+    out["align_ms"] = mean(erec, "align_ms")
+    bins = erec[-1]["bins"]
+    for i, b in enumerate(bins):
+        b["ms"] = float(np.mean([r["bins"][i]["ms"] for r in erec]))
+    out["bins"] = bins
+    assert erec[-1]["stats"]["n_edges"] == out["stats"]["n_edges"]
+
+    if plain_steps:
+        pdt, prec = run(ng.GRAPH_PLAIN, plain_steps, 1)
+        log("%s plain: %.2f ms/step" % (label, 1e3 * pdt / plain_steps))
+        assert prec[-1]["stats"]["n_edges"] == out["stats"]["n_edges"]
+        out["plain"] = {"value": round(d.n * plain_steps / pdt, 1),
+                        "ms_per_step": round(1e3 * pdt / plain_steps, 3),
+                        "steps": plain_steps,
+                        "graph_ms": round(mean(prec, "graph_ms"), 3),
+                        "host_syncs": prec[-1]["stats"]["host_syncs"],
+                        "align_launches": prec[-1]["stats"]["align_launches"],
+                        "edges_identical": True}
+    db.close()
+    return out
+
+
+def align_rooflines(m, peak):
+    rows = []
+    for b in m["bins"]:
+        if b["kind"] not in ("window", "wave", "plain") or not b["pairs"] or b["ms"] <= 0:
+            continue
+        ops = b["ref_tiles"] * OPS_PER_TILE
+        ach = ops / (b["ms"] * 1e-3) / 1e12
+        rows.append({"kernel": kernel_name(b), "pairs": b["pairs"], "ref_tiles": b["ref_tiles"],
+                     "kernel_ms": round(b["ms"], 4), "achieved": round(ach, 3),
+                     "frac": round(ach / peak, 4)})
+    rows.sort(key=lambda r: -r["kernel_ms"])
+    tiles = sum(b["ref_tiles"] for b in m["bins"])
+    phase = tiles * OPS_PER_TILE / (m["align_ms"] * 1e-3) / 1e12 if m["align_ms"] > 0 else 0.0
+    return rows, {"ref_tiles": tiles, "ms": round(m["align_ms"], 4), "achieved": round(phase, 3),
+                  "frac": round(phase / peak, 4)}
+
+
+def cpu_baseline(ng, orc, config, scale, seed, identity):
+    """The oracle's sketch + bucket + graph over a sample of the workload, all host threads."""
     threads = len(os.sched_getaffinity(0))
     d = ng.gen_packed(ng.gen_config(config, scale, seed))
     t0 = time.perf_counter()
@@ -54,10 +164,10 @@ def cpu_baseline(config, scale, seed, identity):
     rep = orc.bucket(sig, threads)
     edges, tot = orc.build_graph(d.packed, d.word_off, d.lengths, rep, 64, identity, 1, threads)
     dt = time.perf_counter() - t0
-    return {"value": d.n / dt, "unit": "seqs/s", "cores": threads, "kind": "port",
-            "sample": "config %d scaled to %d sequences (%g of the families), seed %d, -i %d: "
-                      "sketch + bucket + 64-band graph, %.1f s, %d candidate pairs, %d edges"
-                      % (config, d.n, scale, seed, identity, dt, tot, len(edges))}
+    return {"value": round(d.n / dt, 1), "unit": "seqs/s", "cores": threads, "kind": "port",
+            "sample": "the same generator scaled to %d sequences (%g of the families), seed %d, "
+                      "-i %d: sketch + bucket + 64-band graph in %.1f s, %d candidate pairs, "
+                      "%d edges" % (d.n, scale, seed, identity, dt, tot, len(edges))}
 
 
 def main():
@@ -65,11 +175,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--config", type=int, default=4, help="generator config of the workload")
+    ap.add_argument("--also-config", type=int, default=2,
+                    help="second workload reported beside the first (0: none)")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--identity", type=int, default=90)
-    ap.add_argument("--cpu-scale", type=float, default=1.0)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-scale", type=float, default=0.1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
@@ -83,99 +196,106 @@ def main():
     import numpy as np
     import ngia2_amd as ng
 
-    t0 = time.perf_counter()
-    d = ng.gen_packed(ng.gen_config(args.config, args.scale, args.seed))
-    gen_s = time.perf_counter() - t0
-    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
-    log("generated %d sequences in %.1f s" % (d.n, gen_s))
-
     ctx = ng.Context(0)
     info = ctx.info()
-    db = ng.Database(ctx, d.packed, d.word_off, d.lengths)
-
-    def step(mode):
-        db.sketch()
-        t_s = ctx.timing()["sketch_ms"]
-        db.bucket()
-        t_b = ctx.timing()["bucket_ms"]
-        st = db.graph(args.identity, 64, mode)
-        tm = ctx.timing()
-        return {"sketch_ms": t_s, "bucket_ms": t_b, "graph_ms": tm["graph_ms"],
-                "finalize_ms": tm["finalize_ms"], "stats": st}
-
-    def run(mode, steps, warmup):
-        for _ in range(warmup):
-            step(mode)
-        ctx.sync()
-        rec = []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            rec.append(step(mode))
-        ctx.sync()
-        dt = time.perf_counter() - t0
-        return dt, rec
-
-    dt, rec = run(ng.GRAPH_BATCHED, args.steps, args.warmup)
-    log("batched: %.2f ms/step" % (1e3 * dt / args.steps))
-    mean = lambda k: float(np.mean([r[k] for r in rec]))
-    st = rec[-1]["stats"]
-    sk_ms, bk_ms, gr_ms, fin_ms = mean("sketch_ms"), mean("bucket_ms"), mean("graph_ms"), \
-        mean("finalize_ms")
-    ms_step = 1e3 * dt / args.steps
-    value = d.n * args.steps / dt
-
-    # dominant kernel: the sketch (one kernel, timed with HIP events on the context stream)
-    ops, nbytes = sketch_work(d.lengths)
     peak = valu_peak_tops(info)
-    achieved = ops / (sk_ms * 1e-3) / 1e12
-    roofline = {"kernel": "k_sketch", "bound": "valu", "achieved": round(achieved, 3),
-                "peak": round(peak, 3), "unit": "TOP/s (int32)",
-                "frac": round(achieved / peak, 4), "traffic": None,
-                "hbm_gbps": round(nbytes / (sk_ms * 1e-3) / 1e9, 2),
-                "hbm_frac": round(nbytes / (sk_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 5),
-                "algorithmic_ops": ops, "algorithmic_bytes": nbytes,
-                "kernel_ms": round(sk_ms, 4)}
+    traffic = load_traffic()
+
+    t0 = time.perf_counter()
+    params = ng.gen_config(args.config, args.scale, args.seed)
+    d = ng.gen_packed(params)
+    log("generated %d sequences (config %d) in %.1f s" % (d.n, args.config,
+                                                        time.perf_counter() - t0))
+    plain_steps = 0 if args.no_plain else args.plain_steps
+    m = measure(ng, np, ctx, info, d, args.identity, args.steps, args.warmup, plain_steps,
+                "config %d" % args.config)
+    st = m["stats"]
+    workload = ("%d proteins in %d families, %s lengths (median %g, max %d), -i %d, 64 bands"
+                % (d.n, params.n_families,
+                   "log-normal + Pareto(%.1f) tail" % params.tail_alpha if params.tail_frac > 0
+                   else "log-normal", params.len_a, params.tail_max if params.tail_frac > 0
+                   else params.len_max, args.identity))
+
+    # sketch: one kernel on the context stream
+    ops, nbytes = sketch_work(d.lengths)
+    sk_ms = m["sketch_ms"]
+    sk_ach = ops / (sk_ms * 1e-3) / 1e12
+    sketch_roof = {"kernel": "k_sketch", "bound": "valu", "achieved": round(sk_ach, 3),
+                   "peak": round(peak, 3), "unit": "TOP/s int32", "frac": round(sk_ach / peak, 4),
+                   "traffic": traffic.get("k_sketch", {}).get(workload),
+                   "kernel_ms": round(sk_ms, 4), "algorithmic_ops": ops,
+                   "algorithmic_bytes": nbytes,
+                   "hbm_gbps": round(nbytes / (sk_ms * 1e-3) / 1e9, 2),
+                   "hbm_frac": round(nbytes / (sk_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 5)}
+    align_rows, phase = align_rooflines(m, peak)
+    # the dominant kernel: the one with the most device time per step
+    roofline = sketch_roof
+    if align_rows and align_rows[0]["kernel_ms"] > sk_ms:
+        a = align_rows[0]
+        roofline = {"kernel": a["kernel"], "bound": "valu", "achieved": a["achieved"],
+                    "peak": round(peak, 3), "unit": "TOP/s int32", "frac": a["frac"],
+                    "traffic": traffic.get(a["kernel"], {}).get(workload),
+                    "kernel_ms": a["kernel_ms"], "algorithmic_ops": a["ref_tiles"] * OPS_PER_TILE,
+                    "ref_tiles": a["ref_tiles"], "pairs": a["pairs"]}
 
     out = {"metric": "seqs clustered/s + aligned pairs/s at -i %d (1 GPU); p50/p99 stream latency"
                      % args.identity,
-           "value": round(value, 1), "unit": "seqs/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "int32",
-           "data": "synthetic (seeded generator, config %d)" % args.config,
-           "config": {"workload": "config %d: %d proteins in %d families, median length %g, -i %d, "
-                                  "64 bands, batched form" % (args.config, d.n,
-                                                              ng.gen_config(args.config,
-                                                                            args.scale).n_families,
-                                                              ng.gen_config(args.config).len_a,
-                                                              args.identity),
-                      "sequences": d.n, "identity": args.identity, "bands": 64,
-                      "residues": int(d.lengths.astype(np.int64).sum()), "seed": args.seed},
-           "aligned_pairs_per_s": round(st["total_pairs"] / ((gr_ms + fin_ms) * 1e-3), 1),
-           "distinct_pairs_per_s": round(st["distinct_pairs"] / ((gr_ms + fin_ms) * 1e-3), 1),
+           "value": round(m["value"], 1), "unit": "seqs/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(m["ms_per_step"], 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+           "data": "synthetic (seeded generator, config %d, seed %d)" % (args.config, args.seed),
+           "config": {"workload": workload, "sequences": d.n, "families": params.n_families,
+                      "identity": args.identity, "bands": 64,
+                      "residues": int(d.lengths.astype(np.int64).sum()),
+                      "max_length": int(d.lengths.max()), "seed": args.seed,
+                      "form": "batched (length-binned kernels, captured hipGraph)"},
+           "aligned_pairs_per_s": round(st["total_pairs"] / ((m["graph_ms"] + m["finalize_ms"])
+                                                              * 1e-3), 1),
+           "distinct_pairs_per_s": round(st["distinct_pairs"] / ((m["graph_ms"] + m["finalize_ms"])
+                                                                  * 1e-3), 1),
            "pairs": {"candidates_with_duplicates": st["total_pairs"],
                      "distinct": st["distinct_pairs"], "edges": st["n_edges"],
                      "ref_tiles": st["ref_tiles"]},
-           "stage_ms": {"sketch": round(sk_ms, 3), "bucket": round(bk_ms, 3),
-                        "graph": round(gr_ms, 3), "finalize": round(fin_ms, 3)},
+           "stage_ms": {"sketch": round(m["sketch_ms"], 3), "bucket": round(m["bucket_ms"], 3),
+                        "graph": round(m["graph_ms"], 3), "finalize": round(m["finalize_ms"], 3)},
            "roofline": roofline,
+           "sketch_roofline": sketch_roof,
+           "align_phase": dict(phase, peak=round(peak, 3), unit="TOP/s int32",
+                               eager_ms_per_step=round(m["eager_ms_per_step"], 3)),
+           "align_kernels": align_rows[:8],
            "device": {"name": info["name"], "arch": info["arch"], "cu": info["cu_count"],
                       "clock_mhz": info["clock_khz"] / 1e3}}
+    if "plain" in m:
+        out["plain_form"] = m["plain"]
+        out["batched_vs_plain"] = round(m["value"] / m["plain"]["value"], 3)
 
-    if not args.no_plain:
-        pdt, prec = run(ng.GRAPH_PLAIN, max(1, min(args.steps, 3)), 1)
-        psteps = max(1, min(args.steps, 3))
-        pv = d.n * psteps / pdt
-        out["plain_form"] = {"value": round(pv, 1), "ms_per_step": round(1e3 * pdt / psteps, 3),
-                             "graph_ms": round(float(np.mean([r["graph_ms"] for r in prec])), 3),
-                             "host_syncs": prec[-1]["stats"]["host_syncs"],
-                             "align_launches": prec[-1]["stats"]["align_launches"]}
-        out["batched_vs_plain"] = round(value / pv, 3)
-        log("plain: %.2f ms/step" % (1e3 * pdt / psteps))
+    if args.also_config:
+        p2 = ng.gen_config(args.also_config, args.scale, args.seed)
+        d2 = ng.gen_packed(p2)
+        m2 = measure(ng, np, ctx, info, d2, args.identity, args.steps, args.warmup,
+                     0 if args.no_plain else 3, "config %d" % args.also_config)
+        rows2, phase2 = align_rooflines(m2, peak)
+        o2 = {"workload": "%d proteins in %d families, log-normal lengths (median %g, max %d), "
+                          "-i %d" % (d2.n, p2.n_families, p2.len_a, p2.len_max, args.identity),
+              "value": round(m2["value"], 1), "ms_per_step": round(m2["ms_per_step"], 3),
+              "stage_ms": {"sketch": round(m2["sketch_ms"], 3), "bucket": round(m2["bucket_ms"], 3),
+                           "graph": round(m2["graph_ms"], 3),
+                           "finalize": round(m2["finalize_ms"], 3)},
+              "pairs": {"candidates_with_duplicates": m2["stats"]["total_pairs"],
+                        "distinct": m2["stats"]["distinct_pairs"],
+                        "edges": m2["stats"]["n_edges"], "ref_tiles": m2["stats"]["ref_tiles"]},
+              "align_phase": phase2, "align_kernels": rows2[:4]}
+        if "plain" in m2:
+            o2["plain_form"] = m2["plain"]
+            o2["batched_vs_plain"] = round(m2["value"] / m2["plain"]["value"], 3)
+        out["config_%d" % args.also_config] = o2
 
-    db.close()
     ctx.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.config, args.cpu_scale, args.seed, args.identity)
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as orc
+        out["cpu_baseline"] = cpu_baseline(ng, orc, args.config, args.cpu_scale, args.seed,
+                                           args.identity)
         log("cpu baseline: %.1f seqs/s" % out["cpu_baseline"]["value"])
     print(json.dumps(out), flush=True)
 

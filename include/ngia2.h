@@ -50,6 +50,8 @@ typedef struct {
 /* Graph-build modes (flags of ngia_db_graph / ngia_build_graph). */
 #define NGIA_GRAPH_PLAIN 0u    /* one align launch + host sync per band (buildGraph.hpp:274-350) */
 #define NGIA_GRAPH_BATCHED 1u  /* cross-band dedupe, length-binned kernels, one hipGraph */
+#define NGIA_GRAPH_BATCHED_EAGER 2u /* the batched kernels launched without the graph, with
+                                     * the alignment phase timed (ngia_timing.align_ms) */
 
 typedef struct {
   uint64_t total_pairs;     /* candidates summed over bands, duplicates included ("compare pairs", buildGraph.hpp:353) */
@@ -91,6 +93,7 @@ typedef struct {
   double bucket_ms;   /* transpose + radix sort + predecessor of the last ngia_db_bucket */
   double graph_ms;    /* candidates .. raw edges of the last ngia_db_graph */
   double finalize_ms; /* sort + unique of the last ngia_db_graph */
+  double align_ms;    /* alignment bins alone (NGIA_GRAPH_BATCHED_EAGER only, else 0) */
 } ngia_timing;
 int ngia_get_timing(ngia_ctx* ctx, ngia_timing* out);
 void ngia_free(void* p);
@@ -149,6 +152,26 @@ int ngia_db_get_edges(ngia_ctx* ctx, ngia_db* db, ngia_edge* out, uint64_t cap,
  * keys[i] = rep<<32 | tar.  Either array may be NULL. */
 int ngia_db_get_pair_scores(ngia_ctx* ctx, ngia_db* db, uint64_t* keys, uint32_t* scores,
                             uint64_t cap, uint64_t* n_pairs);
+
+/* Length bins of the last batched graph.  No reference counterpart (the
+ * reference aligns every band's jobs in one kernel_dynamic launch).  A pair's
+ * bin follows from its band window W = lsft/32 + ceil(rsft/32) words:
+ * W <= 16 -> the thread-per-pair window kernel of width W; W <= 64*K -> the
+ * wavefront-per-pair kernel with K words per lane; wider -> the plain kernel;
+ * length-rejected or empty targets score 0 without a kernel. */
+#define NGIA_BIN_ZERO 0u
+#define NGIA_BIN_WINDOW 1u
+#define NGIA_BIN_WAVE 2u
+#define NGIA_BIN_PLAIN 3u
+typedef struct {
+  uint32_t kind;       /* NGIA_BIN_* */
+  uint32_t width;      /* W (window) or K (wave); 0 otherwise */
+  uint64_t pairs;      /* distinct pairs in the bin */
+  uint64_t ref_tiles;  /* reference 32x32 tiles of those pairs */
+  double ms;           /* the bin's kernel time on its stream (NGIA_GRAPH_BATCHED_EAGER), else 0 */
+} ngia_bin_stat;
+int ngia_db_bin_stats(ngia_ctx* ctx, ngia_db* db, ngia_bin_stat* out, uint32_t cap,
+                      uint32_t* n_bins);
 
 /* ------------------------------------------- fast mode (no graph, no Leiden) */
 /* FastClustering::cluster (fastClustering.hpp:249-352): per band, align the

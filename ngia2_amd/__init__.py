@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (BANDS, GRAPH_BATCHED, GRAPH_PLAIN, NGIA_ERR_ARG, NGIA_ERR_IO,
+from ._lib import (BANDS, GRAPH_BATCHED, GRAPH_BATCHED_EAGER, GRAPH_PLAIN, NGIA_ERR_ARG, NGIA_ERR_IO,
                    NGIA_ERR_NODEV, NGIA_ERR_STATE, NGIA_OK, BIN_DIR, LIB_PATH)
 
 __all__ = ["NgiaError", "Context", "Database", "PackedDB", "gen_config", "gen_fasta",
@@ -220,6 +220,16 @@ class Database:
             _check(L.ngia_db_get_pair_scores(self.ctx.h, self.h, _ptr(keys), _ptr(sc), m.value,
                                              C.byref(m)), self.ctx.h)
         return keys, sc
+
+    def bin_stats(self):
+        """Length bins of the last batched graph: kind, width, pairs, ref_tiles, ms."""
+        L = _lib.lib()
+        nb = C.c_uint32()
+        _check(L.ngia_db_bin_stats(self.ctx.h, self.h, None, 0, C.byref(nb)), self.ctx.h)
+        arr = (_lib.BinStat * nb.value)()
+        _check(L.ngia_db_bin_stats(self.ctx.h, self.h, arr, nb.value, C.byref(nb)), self.ctx.h)
+        return [{"kind": _lib.BIN_KINDS[b.kind], "width": b.width, "pairs": b.pairs,
+                 "ref_tiles": b.ref_tiles, "ms": b.ms} for b in arr]
 
     def fast(self, identity, bands=BANDS):
         cl = np.zeros(self.n, dtype=np.uint32)

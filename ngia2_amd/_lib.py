@@ -19,6 +19,7 @@ NGIA_ERR_STATE = -5
 NGIA_ERR_NODEV = -6
 GRAPH_PLAIN = 0
 GRAPH_BATCHED = 1
+GRAPH_BATCHED_EAGER = 2
 BANDS = 64
 
 
@@ -44,9 +45,17 @@ class GraphStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BinStat(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("width", C.c_uint32), ("pairs", C.c_uint64),
+                ("ref_tiles", C.c_uint64), ("ms", C.c_double)]
+
+
+BIN_KINDS = {0: "zero", 1: "window", 2: "wave", 3: "plain"}
+
+
 class Timing(C.Structure):
     _fields_ = [("sketch_ms", C.c_double), ("bucket_ms", C.c_double), ("graph_ms", C.c_double),
-                ("finalize_ms", C.c_double)]
+                ("finalize_ms", C.c_double), ("align_ms", C.c_double)]
 
 
 class DeviceInfo(C.Structure):
@@ -128,6 +137,7 @@ PROTOTYPES = {
     "ngia_db_graph": (C.c_int, [P, P, U32, U32, U32, C.POINTER(GraphStats)]),
     "ngia_db_get_edges": (C.c_int, [P, P, P, U64, C.POINTER(U64)]),
     "ngia_db_get_pair_scores": (C.c_int, [P, P, P, P, U64, C.POINTER(U64)]),
+    "ngia_db_bin_stats": (C.c_int, [P, P, P, U32, C.POINTER(U32)]),
     "ngia_db_fast": (C.c_int, [P, P, U32, U32, P, C.POINTER(U64)]),
     "ngia_leiden": (C.c_int, [P, U64, I32, C.c_float, U32, I32, P]),
     "ngia_resolution": (C.c_float, [U32]),

@@ -61,8 +61,8 @@ struct DevBuf {
 struct Counters {
   unsigned long long total_cand;  // candidates with duplicates
   unsigned long long ecount;      // raw edges
-  unsigned long long tiles;       // reference tiles of the aligned pairs
   unsigned long long unique;      // edges after unique
+  unsigned long long bin_tiles[kBins];  // reference tiles of the pairs in each bin
   uint32_t bin_cnt[kBins];
   uint32_t bin_off[kBins + 1];
   uint32_t cursor[kBins];
@@ -80,6 +80,8 @@ struct ngia_ctx {
   // side streams (default priority) for the concurrent alignment bins
   hipStream_t side[kSideStreams] = {};
   hipEvent_t fork = nullptr, join[kSideStreams] = {};
+  // start/end of each bin's kernel in the eager batched form
+  hipEvent_t bin_ev[2 * kBins] = {};
   uint32_t wave_kmax = 8;  // largest k_align_wave<K> usable on this device
   // NGIA_BUCKET_WIDE=1 selects the 38-bit-key bucketing that n >= 2^26 uses
   bool bucket_wide = false;
@@ -88,6 +90,8 @@ struct ngia_ctx {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : join)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : bin_ev)
       if (e) (void)hipEventDestroy(e);
     if (fork) (void)hipEventDestroy(fork);
     for (auto& q : side)
@@ -111,6 +115,10 @@ struct ngia_db {
   // the captured batched round
   hipGraphExec_t gexec = nullptr;
   uint32_t g_identity = 0, g_bands = 0;
+  // per-bin figures of the last batched graph
+  bool have_bins = false;
+  Counters last{};
+  float bin_ms[kBins] = {};
   ~ngia_db() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
   }
@@ -250,6 +258,7 @@ void graph_plain(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
   hipStream_t s = ctx->stream;
   const uint32_t n = db->n;
   const float thr = (float)identity / 100.0f;
+  db->have_bins = false;
   db->flags.ensure(4ull * n);
   db->pos.ensure(8ull * (n + 1));
   db->sums.ensure(8 * scan_scratch_words(n));
@@ -322,7 +331,8 @@ void graph_plain(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
 }
 
 // Enqueue one batched round (capturable: fixed grids, device-side counts).
-void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands) {
+void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
+                     bool time_align) {
   hipStream_t s = ctx->stream;
   const uint32_t n = db->n;
   const float thr = (float)identity / 100.0f;
@@ -336,39 +346,51 @@ void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ban
   const uint64_t* npairs = db->off.as<uint64_t>() + n;
   const uint32_t grid = persistent_blocks(ctx, 0) * 4;
   bin_pairs(db->pairs.as<uint2>(), npairs, len, thr, db->bin_of.as<uint32_t>(), dc->bin_cnt,
-            dc->bin_off, dc->cursor, db->list.as<uint32_t>(), &dc->tiles, db->score.as<uint32_t>(),
+            dc->bin_off, dc->cursor, db->list.as<uint32_t>(), dc->bin_tiles, db->score.as<uint32_t>(),
             ctx->wave_kmax, grid, s);
   const AlignArgs aa = align_args(db, thr);
   // The bins run concurrently: forked onto the side streams (round-robin,
   // widest windows first since their pairs take longest) and joined back, so
   // a small bin's tail does not leave the GPU idle.
+  if (time_align) NGIA_HIP(hipEventRecord(ctx->ev[2], s));
   NGIA_HIP(hipEventRecord(ctx->fork, s));
   for (int k = 0; k < kSideStreams; k++) NGIA_HIP(hipStreamWaitEvent(ctx->side[k], ctx->fork, 0));
   int slot = 0;
+  auto mark = [&](int bin, int end, hipStream_t q) {
+    if (time_align) NGIA_HIP(hipEventRecord(ctx->bin_ev[2 * bin + end], q));
+  };
   for (int k = 5; k >= 0; k--) {
     if ((1u << k) > ctx->wave_kmax) continue;
     const int bin = kBinWave0 + k;
+    hipStream_t q = ctx->side[slot++ % kSideStreams];
+    mark(bin, 0, q);
     launch_align_wave(1 << k, aa, db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[bin],
                       &dc->bin_cnt[bin], &dc->work[bin], db->score.as<uint32_t>(),
-                      persistent_blocks(ctx, wave_lds_bytes(1 << k)),
-                      ctx->side[slot++ % kSideStreams]);
+                      persistent_blocks(ctx, wave_lds_bytes(1 << k)), q);
+    mark(bin, 1, q);
   }
   for (int W = kWinMax; W >= 2; W--) {
+    hipStream_t q = ctx->side[slot++ % kSideStreams];
+    mark(W, 0, q);
     launch_align_win(W, aa, db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[W],
                      &dc->bin_cnt[W], &dc->work[W], db->score.as<uint32_t>(),
-                     persistent_blocks(ctx, win_lds_bytes(W)), ctx->side[slot++ % kSideStreams]);
+                     persistent_blocks(ctx, win_lds_bytes(W)), q);
+    mark(W, 1, q);
   }
   for (int k = 0; k < kSideStreams; k++) {
     NGIA_HIP(hipEventRecord(ctx->join[k], ctx->side[k]));
     NGIA_HIP(hipStreamWaitEvent(s, ctx->join[k], 0));
   }
+  if (time_align) NGIA_HIP(hipEventRecord(ctx->ev[3], s));
   // windows beyond 64*32 words: the plain kernel over job triplets
   plain_bin_jobs(db->pairs.as<uint2>(), db->list.as<uint32_t>(), dc->bin_off, dc->bin_cnt,
                  db->plain_jobs.as<uint32_t>(), s);
   const uint32_t nw_max = (db->max_len + 31) / 32;
+  mark(kBinPlain, 0, s);
   launch_align_plain(aa, db->plain_jobs.as<uint32_t>(), 0xFFFFFFFFu, nw_max,
                      db->plain_scratch.as<uint32_t>(), db->plain_scratch.bytes / 4, s,
                      &dc->bin_cnt[kBinPlain]);
+  mark(kBinPlain, 1, s);
   plain_bin_scores(db->list.as<uint32_t>(), dc->bin_off, dc->bin_cnt, db->plain_jobs.as<uint32_t>(),
                    db->score.as<uint32_t>(), s);
   pairs_to_edges(db->pairs.as<uint2>(), npairs, db->score.as<uint32_t>(), db->ekeys.as<uint64_t>(),
@@ -376,7 +398,7 @@ void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ban
 }
 
 void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
-                   ngia_graph_stats* st) {
+                   ngia_graph_stats* st, bool eager) {
   hipStream_t s = ctx->stream;
   const uint32_t n = db->n;
   const uint64_t cap = (uint64_t)n * bands;  // distinct pairs <= candidates <= n*bands
@@ -396,7 +418,10 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   ensure_edge_space(db, cap);
   const bool reuse = db->gexec && db->g_identity == identity && db->g_bands == bands;
   NGIA_HIP(hipEventRecord(ctx->ev[0], s));
-  if (!reuse) {
+  if (eager) {
+    // Same kernels without the graph, with events around the alignment phase.
+    enqueue_batched(ctx, db, identity, bands, /*time_align=*/true);
+  } else if (!reuse) {
     if (db->gexec) {
       (void)hipGraphExecDestroy(db->gexec);
       db->gexec = nullptr;
@@ -404,7 +429,7 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
     hipGraph_t g = nullptr;
     NGIA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
-      enqueue_batched(ctx, db, identity, bands);
+      enqueue_batched(ctx, db, identity, bands, false);
     } catch (...) {
       (void)hipStreamEndCapture(s, &g);
       if (g) (void)hipGraphDestroy(g);
@@ -417,7 +442,7 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
     db->g_identity = identity;
     db->g_bands = bands;
   }
-  NGIA_HIP(hipGraphLaunch(db->gexec, s));
+  if (!eager) NGIA_HIP(hipGraphLaunch(db->gexec, s));
   NGIA_HIP(hipEventRecord(ctx->ev[1], s));
   Counters hc;
   d2h(&hc, db->counters.p, sizeof(Counters), s);
@@ -431,11 +456,21 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   NGIA_HIP(hipEventSynchronize(ctx->ev[5]));
   ctx->timing.graph_ms = event_ms(ctx->ev[0], ctx->ev[1]);
   ctx->timing.finalize_ms = event_ms(ctx->ev[4], ctx->ev[5]);
+  ctx->timing.align_ms = eager ? event_ms(ctx->ev[2], ctx->ev[3]) : 0.0;
+  db->last = hc;
+  db->have_bins = true;
+  for (int b = 0; b < kBins; b++) {
+    const bool launched = b == kBinPlain || (b >= 2 && b <= kWinMax) ||
+                          (b >= kBinWave0 && b < kBinPlain && (1u << (b - kBinWave0)) <= ctx->wave_kmax);
+    db->bin_ms[b] = eager && launched ? event_ms(ctx->bin_ev[2 * b], ctx->bin_ev[2 * b + 1]) : 0.f;
+  }
+  uint64_t tiles = 0;
+  for (int b = 0; b < kBins; b++) tiles += hc.bin_tiles[b];
   if (st) {
     st->total_pairs = hc.total_cand;
     st->distinct_pairs = npairs;
     st->n_edges = db->n_edges;
-    st->ref_tiles = hc.tiles;
+    st->ref_tiles = tiles;
     st->ms_candidates = 0;
     st->ms_align = event_ms(ctx->ev[0], ctx->ev[1]);
     st->ms_edges = event_ms(ctx->ev[4], ctx->ev[5]);
@@ -467,6 +502,7 @@ extern "C" int ngia_open(int device, ngia_ctx** out) {
     for (auto& q : ctx->side) NGIA_HIP(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
     NGIA_HIP(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
     for (auto& e : ctx->join) NGIA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& e : ctx->bin_ev) NGIA_HIP(hipEventCreate(&e));
     return NGIA_OK;
   });
   if (rc != NGIA_OK) return rc;
@@ -626,8 +662,8 @@ extern "C" int ngia_db_graph(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint
   if (st) std::memset(st, 0, sizeof(*st));
   return guarded(ctx, [&] {
     auto t0 = std::chrono::steady_clock::now();
-    if (flags == NGIA_GRAPH_BATCHED)
-      graph_batched(ctx, db, identity, bands, st);
+    if (flags == NGIA_GRAPH_BATCHED || flags == NGIA_GRAPH_BATCHED_EAGER)
+      graph_batched(ctx, db, identity, bands, st, flags == NGIA_GRAPH_BATCHED_EAGER);
     else
       graph_plain(ctx, db, identity, bands, st, nullptr);
     auto t1 = std::chrono::steady_clock::now();
@@ -682,6 +718,35 @@ extern "C" int ngia_db_get_pair_scores(ngia_ctx* ctx, ngia_db* db, uint64_t* key
     }
     return NGIA_OK;
   });
+}
+
+extern "C" int ngia_db_bin_stats(ngia_ctx* ctx, ngia_db* db, ngia_bin_stat* out, uint32_t cap,
+                                 uint32_t* n_bins) {
+  if (!ctx || !db || !n_bins) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  if (!db->have_bins) return fail(ctx, NGIA_ERR_STATE, "no batched graph has run on this database");
+  *n_bins = kBins;
+  if (!out) return NGIA_OK;
+  if (cap < (uint32_t)kBins) return fail(ctx, NGIA_ERR_ARG, "bin buffer too small");
+  for (int b = 0; b < kBins; b++) {
+    ngia_bin_stat& o = out[b];
+    if (b == kBinZero) {
+      o.kind = NGIA_BIN_ZERO;
+      o.width = 0;
+    } else if (b <= kWinMax) {
+      o.kind = NGIA_BIN_WINDOW;
+      o.width = (uint32_t)b;
+    } else if (b < kBinPlain) {
+      o.kind = NGIA_BIN_WAVE;
+      o.width = 1u << (b - kBinWave0);
+    } else {
+      o.kind = NGIA_BIN_PLAIN;
+      o.width = 0;
+    }
+    o.pairs = db->last.bin_cnt[b];
+    o.ref_tiles = db->last.bin_tiles[b];
+    o.ms = db->bin_ms[b];
+  }
+  return NGIA_OK;
 }
 
 extern "C" int ngia_db_fast(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,

@@ -310,29 +310,31 @@ __device__ inline uint32_t pair_bin(uint32_t R, uint32_t T, float thr, uint32_t 
 __global__ void k_bin_count(const uint2* __restrict__ pairs, const uint64_t* __restrict__ npairs,
                             const uint32_t* __restrict__ len, float thr, uint32_t kmax,
                             uint32_t* __restrict__ bin_of, uint32_t* __restrict__ bin_cnt,
-                            unsigned long long* __restrict__ tiles_total,
+                            unsigned long long* __restrict__ bin_tiles,
                             uint32_t* __restrict__ score) {
   const uint64_t P = *npairs;
   __shared__ uint32_t local[kBins];
-  __shared__ unsigned long long ltiles;
-  if (threadIdx.x < kBins) local[threadIdx.x] = 0;
-  if (threadIdx.x == 0) ltiles = 0;
+  __shared__ unsigned long long ltiles[kBins];
+  if (threadIdx.x < kBins) {
+    local[threadIdx.x] = 0;
+    ltiles[threadIdx.x] = 0;
+  }
   __syncthreads();
-  uint64_t tsum = 0;
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
        p += (uint64_t)gridDim.x * blockDim.x) {
     const uint2 pr = pairs[p];
     uint64_t t;
     const uint32_t b = pair_bin(len[pr.x], len[pr.y], thr, kmax, &t);
     bin_of[p] = b;
-    tsum += t;
     if (b == kBinZero) score[p] = 0;
     atomicAdd(&local[b], 1u);
+    if (t) atomicAdd(&ltiles[b], (unsigned long long)t);
   }
-  atomicAdd(&ltiles, (unsigned long long)tsum);
   __syncthreads();
-  if (threadIdx.x < kBins && local[threadIdx.x]) atomicAdd(&bin_cnt[threadIdx.x], local[threadIdx.x]);
-  if (threadIdx.x == 0 && ltiles) atomicAdd(tiles_total, ltiles);
+  if (threadIdx.x < kBins && local[threadIdx.x]) {
+    atomicAdd(&bin_cnt[threadIdx.x], local[threadIdx.x]);
+    atomicAdd(&bin_tiles[threadIdx.x], ltiles[threadIdx.x]);
+  }
 }
 
 // bin_off[b] = exclusive prefix of bin_cnt; also resets the scatter cursors.
@@ -384,10 +386,10 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const uint64_t* __restrict_
 
 void bin_pairs(const uint2* pairs, const uint64_t* npairs, const uint32_t* len, float thr,
                uint32_t* bin_of, uint32_t* bin_cnt, uint32_t* bin_off, uint32_t* cursor,
-               uint32_t* list, unsigned long long* tiles_total, uint32_t* score, uint32_t kmax,
+               uint32_t* list, unsigned long long* bin_tiles, uint32_t* score, uint32_t kmax,
                uint32_t grid, hipStream_t s) {
   hipLaunchKernelGGL(k_bin_count, dim3(grid), dim3(256), 0, s, pairs, npairs, len, thr, kmax, bin_of,
-                     bin_cnt, tiles_total, score);
+                     bin_cnt, bin_tiles, score);
   hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, bin_cnt, bin_off, cursor);
   hipLaunchKernelGGL(k_bin_scatter, dim3(grid), dim3(256), 0, s, npairs, bin_of, bin_off, cursor,
                      list);

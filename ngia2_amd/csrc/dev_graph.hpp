@@ -33,7 +33,7 @@ void pair_emit(const uint32_t* rep, const uint32_t* len, uint32_t n, uint32_t ba
                const uint64_t* off, uint2* pairs, hipStream_t s);
 void bin_pairs(const uint2* pairs, const uint64_t* npairs, const uint32_t* len, float thr,
                uint32_t* bin_of, uint32_t* bin_cnt, uint32_t* bin_off, uint32_t* cursor,
-               uint32_t* list, unsigned long long* tiles_total, uint32_t* score, uint32_t kmax,
+               uint32_t* list, unsigned long long* bin_tiles, uint32_t* score, uint32_t kmax,
                uint32_t grid, hipStream_t s);
 void plain_bin_jobs(const uint2* pairs, const uint32_t* list, const uint32_t* bin_off,
                     const uint32_t* bin_cnt, uint32_t* jobs, hipStream_t s);

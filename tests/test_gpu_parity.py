@@ -253,3 +253,45 @@ def test_graph_replay(ctx, c1db):
         np.testing.assert_array_equal(db.edges(), want)
         db.graph(70, 64, ng.GRAPH_PLAIN)  # eager work between replays
         np.testing.assert_array_equal(db.edges(), want)
+
+
+@pytest.mark.parametrize("identity", [50, 90])
+def test_eager_batched_and_bin_accounting(ctx, longdb, identity):
+    """The uncaptured batched form scores like the captured one; every distinct pair
+    sits in exactly one bin, and each bin's tile count is the oracle's reference
+    tile count over that bin's pairs."""
+    _, _, d = longdb
+    n = d["n"]
+    rows = []
+    for b in range(64):
+        r = np.arange(n, dtype=np.int64) - 1 - b
+        r[r < 0] = np.arange(n)[r < 0]
+        rows.append(r.astype(np.uint32))
+    rep = np.stack(rows)
+    say("eager batched identity", identity)
+    db = ng.Database(ctx, d["packed"], d["word_off"], d["lengths"])
+    db.set_reps(rep, 64)
+    st_g = db.graph(identity, 64, ng.GRAPH_BATCHED)
+    keys_g, sc_g = db.pair_scores()
+    edges_g = db.edges()
+    st_e = db.graph(identity, 64, ng.GRAPH_BATCHED_EAGER)
+    keys_e, sc_e = db.pair_scores()
+    np.testing.assert_array_equal(keys_e, keys_g)
+    np.testing.assert_array_equal(sc_e, sc_g)
+    np.testing.assert_array_equal(db.edges(), edges_g)
+    assert st_e["ref_tiles"] == st_g["ref_tiles"]
+    assert ctx.timing()["align_ms"] > 0
+    bins = db.bin_stats()
+    assert sum(b["pairs"] for b in bins) == st_e["distinct_pairs"] == len(keys_e)
+    assert sum(b["ref_tiles"] for b in bins) == st_e["ref_tiles"]
+    thr = float(np.float32(identity) / np.float32(100))
+    lens = d["lengths"].astype(np.int64)
+    R = lens[(keys_e >> np.uint64(32)).astype(np.int64)]
+    T = lens[(keys_e & np.uint64(0xFFFFFFFF)).astype(np.int64)]
+    tiles = np.array([orc.align_tiles(int(r), int(t), thr) for r, t in zip(R, T)])
+    assert int(tiles.sum()) == st_e["ref_tiles"]
+    used = [b for b in bins if b["pairs"]]
+    assert any(b["kind"] == "wave" for b in used) and any(b["kind"] == "window" for b in used)
+    for b in used:
+        if b["kind"] in ("window", "wave", "plain"):
+            assert b["ms"] > 0, b
