@@ -85,6 +85,9 @@ struct ngia_ctx {
   uint32_t wave_kmax = 8;  // largest k_align_wave<K> usable on this device
   // NGIA_BUCKET_WIDE=1 selects the 38-bit-key bucketing that n >= 2^26 uses
   bool bucket_wide = false;
+  // NGIA_SERIAL_BINS=1 runs the alignment bins one after another on the
+  // context stream (each bin's kernel time is then its time alone)
+  bool serial_bins = false;
   ngia_timing timing{};    // device time of the last sketch / bucket / graph
   ~ngia_ctx() {
     for (auto& e : ev)
@@ -349,20 +352,27 @@ void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ban
             dc->bin_off, dc->cursor, db->list.as<uint32_t>(), dc->bin_tiles, db->score.as<uint32_t>(),
             ctx->wave_kmax, grid, s);
   const AlignArgs aa = align_args(db, thr);
-  // The bins run concurrently: forked onto the side streams (round-robin,
-  // widest windows first since their pairs take longest) and joined back, so
-  // a small bin's tail does not leave the GPU idle.
+  // The bins run concurrently, forked onto the side streams and joined back.
+  // The pairs of the wide wave bins are the longest, so the bins start
+  // widest first: K >= 8 on side stream 0, K = 4 alone on side stream 1
+  // (in heavy-tailed sets it holds the longest pairs), K = 2 and 1 on side
+  // stream 2, the window bins spread over streams 3, 0 and 2.
   if (time_align) NGIA_HIP(hipEventRecord(ctx->ev[2], s));
-  NGIA_HIP(hipEventRecord(ctx->fork, s));
-  for (int k = 0; k < kSideStreams; k++) NGIA_HIP(hipStreamWaitEvent(ctx->side[k], ctx->fork, 0));
-  int slot = 0;
+  const bool fork = !ctx->serial_bins;
+  if (fork) {
+    NGIA_HIP(hipEventRecord(ctx->fork, s));
+    for (int k = 0; k < kSideStreams; k++) NGIA_HIP(hipStreamWaitEvent(ctx->side[k], ctx->fork, 0));
+  }
   auto mark = [&](int bin, int end, hipStream_t q) {
     if (time_align) NGIA_HIP(hipEventRecord(ctx->bin_ev[2 * bin + end], q));
   };
+  auto wave_stream = [&](int k) { return fork ? ctx->side[k >= 3 ? 0 : k == 2 ? 1 : 2] : s; };
+  static constexpr int kWinStreams[3] = {3, 0, 2};
+  int slot = 0;
   for (int k = 5; k >= 0; k--) {
     if ((1u << k) > ctx->wave_kmax) continue;
     const int bin = kBinWave0 + k;
-    hipStream_t q = ctx->side[slot++ % kSideStreams];
+    hipStream_t q = wave_stream(k);
     mark(bin, 0, q);
     launch_align_wave(1 << k, aa, db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[bin],
                       &dc->bin_cnt[bin], &dc->work[bin], db->score.as<uint32_t>(),
@@ -370,14 +380,14 @@ void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ban
     mark(bin, 1, q);
   }
   for (int W = kWinMax; W >= 2; W--) {
-    hipStream_t q = ctx->side[slot++ % kSideStreams];
+    hipStream_t q = fork ? ctx->side[kWinStreams[slot++ % 3]] : s;
     mark(W, 0, q);
     launch_align_win(W, aa, db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[W],
                      &dc->bin_cnt[W], &dc->work[W], db->score.as<uint32_t>(),
                      persistent_blocks(ctx, win_lds_bytes(W)), q);
     mark(W, 1, q);
   }
-  for (int k = 0; k < kSideStreams; k++) {
+  for (int k = 0; fork && k < kSideStreams; k++) {
     NGIA_HIP(hipEventRecord(ctx->join[k], ctx->side[k]));
     NGIA_HIP(hipStreamWaitEvent(s, ctx->join[k], 0));
   }
@@ -497,6 +507,8 @@ extern "C" int ngia_open(int device, ngia_ctx** out) {
     ctx->wave_kmax = (uint32_t)align_kernels_setup(ctx->prop.sharedMemPerBlock);
     const char* bw = std::getenv("NGIA_BUCKET_WIDE");
     ctx->bucket_wide = bw && bw[0] == '1';
+    const char* sb = std::getenv("NGIA_SERIAL_BINS");
+    ctx->serial_bins = sb && sb[0] == '1';
     NGIA_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (auto& e : ctx->ev) NGIA_HIP(hipEventCreate(&e));
     for (auto& q : ctx->side) NGIA_HIP(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));

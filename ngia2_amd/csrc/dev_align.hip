@@ -252,7 +252,6 @@ __global__ __launch_bounds__(64) void k_align_win(AlignArgs A, const uint2* __re
   // Peq slots [W][16 codes][64 lanes] + staged target codes [8][64 lanes].
   // Static so that its LDS address is a compile-time constant.
   __shared__ uint32_t lds[W * 1024 + 8 * 64];
-  uint32_t* const cst = lds + W * 1024;
   const uint32_t lane = threadIdx.x;
   const uint32_t lane4 = lane * 4;
   #pragma unroll
@@ -304,11 +303,54 @@ __global__ __launch_bounds__(64) void k_align_win(AlignArgs A, const uint2* __re
 }
 
 // ------------------------------------------------ one wavefront per pair
-// Relative window word j = lane*K + m lives in rows[m] of that lane.  The
-// Peq table is a circular buffer of S = 64K word slots per code (plus K
-// mirrored slots so a lane's K consecutive reads never wrap); a word's masks
-// are written when it enters the band and cleared when it leaves, so slots
-// of words above the band read as 0.
+// The wave holds the 64K consecutive rep words [B, B + 64K): lane l keeps
+// words B + lK + m in rows[m].  B = K * floor((cb - a) / K) trails the band
+// bottom: when the bottom crosses a multiple of K the registers shift down
+// one lane (lane 0's K words leave, frozen for good), so the band
+// [cb - a, cb + c) lies inside the held range whenever W + K - 1 <= 64K.
+// Held words outside the band see match mask 0: below the band they are
+// frozen words (mask 0 and carry-in 0 leave a word unchanged and pass carry
+// 0 up), above it untouched all-ones words (a fixed point for either
+// carry-in, absorbing the carry the reference drops at the band top).
+//
+// Match masks: peq[code * S + (w mod S)], S = 64K slots per code.  A word's
+// 15 masks are written by lanes 1..15 when it enters the band and cleared
+// when it leaves; a lane's K slots are contiguous and K-aligned, so they load
+// as one ds_read_b32 / b64 / b128 (or K/4 of the latter) per character.
+// Target codes of the next block and the bit planes of the next entering
+// word are loaded one block ahead.  The block is one wavefront, whose LDS
+// accesses execute in program order, so no workgroup barrier is needed; the
+// empty asm with a memory clobber keeps the compiler from reordering them.
+__device__ inline void lds_order() { asm volatile("" ::: "memory"); }
+
+// Match mask of code k (1..15) over one word from its four bit planes.
+__device__ inline uint32_t code_mask(uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3,
+                                     uint32_t k) {
+  const uint32_t x0 = (k & 1u) ? p0 : ~p0, x1 = (k & 2u) ? p1 : ~p1;
+  const uint32_t x2 = (k & 4u) ? p2 : ~p2, x3 = (k & 8u) ? p3 : ~p3;
+  return x0 & x1 & x2 & x3;
+}
+
+template <int K>
+__device__ inline void load_masks(const uint32_t* p, uint32_t (&M)[K]) {
+  if constexpr (K == 1) {
+    M[0] = p[0];
+  } else if constexpr (K == 2) {
+    const uint2 v = *(const uint2*)p;
+    M[0] = v.x;
+    M[1] = v.y;
+  } else {
+    #pragma unroll
+    for (int q = 0; q < K / 4; q++) {
+      const uint4 v = ((const uint4*)p)[q];
+      M[4 * q] = v.x;
+      M[4 * q + 1] = v.y;
+      M[4 * q + 2] = v.z;
+      M[4 * q + 3] = v.w;
+    }
+  }
+}
+
 template <int K>
 __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __restrict__ pairs,
                                                    const uint32_t* __restrict__ list,
@@ -316,12 +358,15 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
                                                    const uint32_t* __restrict__ count_ptr,
                                                    uint32_t* __restrict__ work,
                                                    uint32_t* __restrict__ score) {
-  constexpr uint32_t S = 64 * K;       // circular slots
-  constexpr uint32_t ST = S + K;       // row stride per code (with mirror)
-  extern __shared__ uint32_t peq[];    // [16][ST]
+  constexpr uint32_t S = 64 * K;  // slots per code
+  constexpr int LG = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : K == 8 ? 3 : K == 16 ? 4 : 5;
+  extern __shared__ __align__(16) uint32_t peq[];  // [16][S]
   const uint32_t lane = threadIdx.x;
   const uint32_t count = *count_ptr;
   const uint32_t* const lst = list + (off_ptr ? *off_ptr : 0u);
+  // The wider the window, the longer the pair: such waves are the critical
+  // path of the concurrent bins, so the SIMD issues them first.
+  __builtin_amdgcn_s_setprio(K >= 4 ? 3 : K == 2 ? 2 : 1);
   for (;;) {
     uint32_t it = 0;
     if (lane == 0) it = atomicAdd(work, 1u);
@@ -337,51 +382,51 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
     const int32_t c = (int32_t)((bd.rsft + 31) / 32);
     const int32_t nw = (int32_t)((R + 31) / 32);
     const uint32_t nb = (T + 31) / 32;
-    const uint8_t* tc = A.codes + A.code_off[pr.y];
+    const uint32_t* tcw = (const uint32_t*)(A.codes + A.code_off[pr.y]);
     // Clear the table, then write the words in the band at block 0: [0, min(c, nw)).
-    for (uint32_t x = lane; x < 16 * ST; x += 64) peq[x] = 0;
-    __syncthreads();
-    auto slot_of = [&](int32_t w) -> uint32_t { return (uint32_t)(w + 64 * S) % S; };
-    auto write_word = [&](int32_t w, bool clear) {
-      // lanes 0..15 each write code `lane` of word w (code 0 stays 0)
-      if (lane >= 1 && lane < 16) {
-        uint32_t v = 0;
-        if (!clear) {
-          const uint32_t* pl = rr + 1 + 4 * w;
-          uint32_t m[16];
-          peq_masks(pl[0], pl[1], pl[2], pl[3], m);
-          #pragma unroll
-          for (int k = 1; k < 16; k++) v = (lane == (uint32_t)k) ? m[k] : v;
-        }
-        const uint32_t s = slot_of(w);
-        peq[lane * ST + s] = v;
-        if (s < K) peq[lane * ST + S + s] = v;
-      }
-    };
+    for (uint32_t x = lane; x < 16 * S / 4; x += 64) ((uint4*)peq)[x] = make_uint4(0u, 0u, 0u, 0u);
+    lds_order();
     const int32_t top0 = c < nw ? c : nw;
-    for (int32_t w = 0; w < top0; w++) write_word(w, false);
-    __syncthreads();
+    for (int32_t w = (int32_t)lane; w < top0; w += 64) {
+      const uint32_t* pl = rr + 1 + 4 * w;
+      uint32_t m[16];
+      peq_masks(pl[0], pl[1], pl[2], pl[3], m);
+      #pragma unroll
+      for (int k = 1; k < 16; k++) peq[k * S + (uint32_t)w % S] = m[k];
+    }
+    lds_order();
     uint32_t rows[K];
     #pragma unroll
     for (int m = 0; m < K; m++) rows[m] = 0xFFFFFFFFu;
+    int32_t B = ((-a) >> LG) * K;  // K * floor(-a / K)
+    uint32_t vs = (uint32_t)(B + (int32_t)(lane * K) + (int32_t)(64 * S)) % S;
     uint32_t lcs = 0;
+    uint32_t cur = (lane < 8 && nb > 0) ? tcw[lane] : 0u;
+    // Wait for `cur` here, so that the loop below never has to wait on the
+    // next block's codes while they are in flight.
+    asm volatile("" : "+v"(cur));
+    // bit planes of the word entering at the end of the current block
+    uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+    if (c < nw) {
+      const uint32_t* pl = rr + 1 + 4 * c;
+      q0 = pl[0]; q1 = pl[1]; q2 = pl[2]; q3 = pl[3];
+    }
     for (uint32_t cb = 0; cb < nb; cb++) {
-      const uint32_t vbase = slot_of((int32_t)cb - a + (int32_t)(lane * K));
-      const uint32_t* tcb = (const uint32_t*)(tc + 32 * cb);
+      const uint32_t nxt = (lane < 8 && cb + 1 < nb) ? tcw[8 * (cb + 1) + lane] : 0u;
+      const uint32_t* pqb = peq + vs;
       for (int g = 0; g < 8; g++) {
-        const uint32_t cw = __builtin_amdgcn_readfirstlane(tcb[g]);
+        const uint32_t cw = __builtin_amdgcn_readlane(cur, g);
+        // the masks of the four characters are loaded before any is used
+        uint32_t Mq[4][K];
+        #pragma unroll
+        for (int mm = 0; mm < 4; mm++) load_masks<K>(pqb + ((cw >> (8 * mm)) & 0xFu) * S, Mq[mm]);
         #pragma unroll
         for (int mm = 0; mm < 4; mm++) {
-          const uint32_t code = (cw >> (8 * mm)) & 0xFu;
-          const uint32_t* pq = peq + code * ST + vbase;
-          uint32_t M[K], t0[K], s0[K];
+          uint32_t(&M)[K] = Mq[mm];
+          uint32_t s0[K];
           uint32_t cy = 0;
           #pragma unroll
-          for (int m = 0; m < K; m++) {
-            M[m] = pq[m];
-            t0[m] = rows[m] & M[m];
-            s0[m] = __builtin_addc(rows[m], t0[m], cy, &cy);
-          }
+          for (int m = 0; m < K; m++) s0[m] = __builtin_addc(rows[m], rows[m] & M[m], cy, &cy);
           uint32_t all = s0[0];
           #pragma unroll
           for (int m = 1; m < K; m++) all &= s0[m];
@@ -393,23 +438,35 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
           for (int m = 0; m < K; m++) {
             const uint32_t s = __builtin_addc(s0[m], 0u, ci, &ci);
             rows[m] = s | (rows[m] & ~M[m]);
+            asm("" : "+v"(rows[m]));  // as in lcs_step: one bitop3, not two
           }
         }
       }
-      // Slide the window one word up.
-      const uint32_t leaving = __builtin_amdgcn_readfirstlane(rows[0]);
-      lcs += 32u - __popc(leaving);
-      const uint32_t up = __shfl_down(rows[0], 1);
-      #pragma unroll
-      for (int m = 0; m < K - 1; m++) rows[m] = rows[m + 1];
-      rows[K - 1] = lane == 63 ? 0xFFFFFFFFu : up;
-      const int32_t wl = (int32_t)cb - a;      // leaving the band
-      const int32_t we = (int32_t)cb + c;      // entering the band at cb + 1
-      __syncthreads();
-      if (wl >= 0 && wl < nw) write_word(wl, true);
-      __syncthreads();
-      if (we < nw) write_word(we, false);
-      __syncthreads();
+      // Word cb - a leaves the band, word cb + c enters it for block cb + 1.
+      const int32_t wl = (int32_t)cb - a, we = (int32_t)cb + c;
+      if (lane >= 1 && lane < 16) {
+        if (wl >= 0 && wl < nw) peq[lane * S + (uint32_t)wl % S] = 0u;
+        if (we < nw) peq[lane * S + (uint32_t)we % S] = code_mask(q0, q1, q2, q3, lane);
+      }
+      lds_order();
+      if (we + 1 < nw) {
+        const uint32_t* pl = rr + 1 + 4 * (we + 1);
+        q0 = pl[0]; q1 = pl[1]; q2 = pl[2]; q3 = pl[3];
+      }
+      // Trail the band bottom of block cb + 1.
+      const int32_t Bn = (((int32_t)cb + 1 - a) >> LG) * K;
+      if (Bn != B) {  // Bn == B + K: lane 0's words leave the held range
+        #pragma unroll
+        for (int m = 0; m < K; m++) {
+          lcs += 32u - __popc(__builtin_amdgcn_readfirstlane(rows[m]));
+          const uint32_t up = __shfl_down(rows[m], 1);
+          rows[m] = lane == 63 ? 0xFFFFFFFFu : up;
+        }
+        B = Bn;
+        vs = (vs + K) % S;
+      }
+      cur = nxt;
+      asm volatile("" : "+v"(cur));
     }
     uint32_t part = 0;
     #pragma unroll
@@ -417,13 +474,13 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
     for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
     lcs += part;
     if (lane == 0) score[p] = lcs_score(lcs, R, T, A.thr);
-    __syncthreads();
+    lds_order();
   }
 }
 
 // ------------------------------------------------ launchers for the bins
 size_t win_lds_bytes(int W) { return (size_t)W * 4096 + 8 * 256; }
-size_t wave_lds_bytes(int K) { return (size_t)16 * (64 * K + K) * 4; }
+size_t wave_lds_bytes(int K) { return (size_t)16 * 64 * K * 4; }
 
 #define NGIA_WIN_CASE(Wv)                                                                  \
   case Wv:                                                                                 \

@@ -107,6 +107,6 @@ void launch_align_wave(int K, const AlignArgs& a, const uint2* pairs, const uint
 // returns the largest K of k_align_wave<K> the device accepts.
 int align_kernels_setup(size_t lds_per_block);
 constexpr int kWinMax = 16;    // window widths 2..16 use k_align_win
-constexpr int kWaveKMax = 32;  // k_align_wave handles W <= 64*32
+constexpr int kWaveKMax = 32;  // k_align_wave handles W + K - 1 <= 64*K, K <= 32
 
 }  // namespace ngia

@@ -280,7 +280,8 @@ void pair_emit(const uint32_t* rep, const uint32_t* len, uint32_t n, uint32_t ba
 
 // Bin of a pair by its band window: W = a + c words.  Bins 2..kWinMax go to
 // k_align_win<W>; larger windows to k_align_wave<K> with the smallest K in
-// {1,2,4,...,32} with 64K >= W; T == 0 scores 0 at once (its LCS is 0, so the
+// {1,2,4,...,32} with 64K >= W + K - 1 (the wave trails the band bottom in
+// steps of K words); T == 0 scores 0 at once (its LCS is 0, so the
 // pass test fails); anything wider goes to the plain kernel.
 __device__ inline uint32_t pair_bin(uint32_t R, uint32_t T, float thr, uint32_t kmax,
                                     uint64_t* tiles) {
@@ -303,7 +304,7 @@ __device__ inline uint32_t pair_bin(uint32_t R, uint32_t T, float thr, uint32_t 
   *tiles = t;
   if (W <= (uint32_t)kWinMax) return W;  // W >= 2 whenever T > 0
   for (uint32_t k = 0, K = 1; K <= kmax; k++, K <<= 1)
-    if (W <= 64u * (uint32_t)K) return kBinWave0 + k;
+    if (W + K - 1 <= 64u * (uint32_t)K) return kBinWave0 + k;
   return kBinPlain;
 }
 
