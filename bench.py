@@ -74,6 +74,22 @@ def load_traffic():
         return json.load(f)
 
 
+def copy_probe(nbytes):
+    """One device-to-device copy of nbytes: a kernel of known HBM traffic (nbytes
+    read + nbytes written), for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("copy probe: %d bytes device to device" % nbytes)
+
+
 def measure(ng, np, ctx, info, d, identity, steps, warmup, plain_steps, label):
     db = ng.Database(ctx, d.packed, d.word_off, d.lengths)
 
@@ -185,6 +201,8 @@ def main():
     ap.add_argument("--cpu-scale", type=float, default=0.1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="bytes of one device-to-device copy run first (counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -196,6 +214,8 @@ def main():
     import numpy as np
     import ngia2_amd as ng
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     ctx = ng.Context(0)
     info = ctx.info()
     peak = valu_peak_tops(info)
