@@ -180,6 +180,73 @@ int ngia_db_bin_stats(ngia_ctx* ctx, ngia_db* db, ngia_bin_stat* out, uint32_t c
 int ngia_db_fast(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
                  uint32_t* cluster, uint64_t* total_pairs);
 
+/* ------------------------------------------- serving */
+/* No reference counterpart: the reference only clusters offline.  A serving
+ * handle keeps a clustered database resident -- packed records, byte codes,
+ * the 64 signatures of every sequence and its cluster id -- and assigns
+ * streamed query sequences to clusters.
+ *
+ * Assignment of one query (fixed by the resident snapshot alone, so it is the
+ * same whether the query is served alone, in a mixed batch or offline):
+ *  1. sketch the query as makeDB sketches a record (makedb.hpp:224-247);
+ *  2. per band, the candidate is the bucket head of the query's signature:
+ *     the lowest resident index with that band signature (the row that
+ *     groupSignatures makes every member point back to, makedb.hpp:340-347);
+ *  3. each distinct candidate forms a pair with the longer sequence as rep
+ *     (ties: the resident), kept if (float)len[tar] >= (float)len[rep]*thr
+ *     (the isJob test, buildGraph.hpp:262-266), and is scored by the
+ *     kernel_dynamic recurrence (buildGraph.hpp:192-253);
+ *  4. the query joins cluster_of[r] of the best-scoring resident r (highest
+ *     score, then lowest index); with no passing pair it opens a new
+ *     cluster numbered new_base + its query id.
+ * Requests wait in arrival order and are admitted into batches as lanes free
+ * up; a lane's capacity in queries and residues is fixed at creation from
+ * the free HBM.  Lanes run on their own streams, so one batch's host packing
+ * and copies overlap another's kernels. */
+#define NGIA_NO_RESIDENT 0xFFFFFFFFu
+typedef struct {
+  uint32_t cluster;     /* cluster_of[resident], or new_base + query id */
+  uint32_t resident;    /* best resident index, NGIA_NO_RESIDENT when a new cluster opens */
+  uint32_t score;       /* its score 1..100, 0 for a new cluster */
+  uint32_t candidates;  /* distinct residents aligned (after the length test) */
+} ngia_assignment;
+typedef struct {
+  uint32_t lanes;          /* batches in flight at once (streams); 0 -> 2 */
+  uint32_t max_queries;    /* queries per batch; 0 -> 8192 */
+  uint64_t max_residues;   /* residues per batch; 0 -> from free HBM, at most 2^26 */
+  double hbm_fraction;     /* share of the free HBM all lanes may take; 0 -> 0.25 */
+} ngia_serve_config;
+typedef struct {
+  double arrival;          /* seconds after the start of the replay (non-decreasing) */
+  uint32_t first, count;   /* queries [first, first + count) of the query set */
+} ngia_request;
+typedef struct {
+  uint64_t queries, requests, batches, pairs, new_clusters;
+  uint32_t lanes, max_queries;   /* lane shape in effect */
+  uint64_t max_residues;
+  double wall_s;           /* start of the replay to the last completion */
+  double device_s;         /* summed device time of the batches (HIP events) */
+} ngia_serve_stats;
+/* Resident snapshot from db (which must be sketched: ngia_db_sketch) and
+ * cluster_of[db n]; identity 1..99. */
+int ngia_serve_create(ngia_ctx* ctx, ngia_db* db, const uint32_t* cluster_of, uint32_t identity,
+                      uint32_t new_base, const ngia_serve_config* cfg, ngia_serve** out);
+void ngia_serve_destroy(ngia_serve* sv);
+/* Queries are residue strings (FASTA letters): query q is
+ * res[res_off[q] .. res_off[q] + res_len[q]), res_len[q] < 65535; its query id
+ * is first_qid + q. */
+int ngia_serve_assign(ngia_serve* sv, const uint8_t* res, const uint64_t* res_off,
+                      const uint32_t* res_len, uint32_t n_queries, uint64_t first_qid,
+                      ngia_assignment* out, ngia_serve_stats* stats);
+/* Real-time replay of a request stream (continuous batching): request r
+ * arrives at reqs[r].arrival; latency_s[r] = its completion (all its queries'
+ * results in host memory) minus its arrival.  Query ids are the query
+ * indices. */
+int ngia_serve_replay(ngia_serve* sv, const uint8_t* res, const uint64_t* res_off,
+                      const uint32_t* res_len, uint32_t n_queries, const ngia_request* reqs,
+                      uint32_t n_reqs, ngia_assignment* out, double* latency_s,
+                      ngia_serve_stats* stats);
+
 /* ------------------------------------------- host side (no device work) */
 /* Leiden over the edge list (LeidenOmp::cluster, leidenParallel.hpp:265-322). */
 int ngia_leiden(const ngia_edge* edges, uint64_t m, int32_t n, float resolution,
@@ -241,6 +308,23 @@ int64_t ngia_gen_fasta(const ngia_gen_params* p, const char* path);
 /* Same sequences, packed in memory in index order (length-descending stable
  * sort of the FASTA order, exactly as makeDB would order them). */
 int ngia_gen_packed(const ngia_gen_params* p, ngia_packed_db* out);
+/* Serving queries: a share `mutated` of them are resident sequences (chosen
+ * uniformly; residues spelled back from their codes) with substitutions at a
+ * rate ~ U[sub_lo, sub_hi] and single-residue indels at rate `indel`; the rest
+ * are uniform random proteins with log-normal lengths (median len_median,
+ * sigma len_sigma, clipped to [len_min, len_max]).  Arrays are malloc'd
+ * (release with ngia_free); kind[q] = 1 for a mutated resident, 0 for novel. */
+typedef struct {
+  uint32_t n_queries;
+  double mutated;
+  double sub_lo, sub_hi, indel;
+  double len_median, len_sigma;
+  uint32_t len_min, len_max;
+  uint64_t seed;
+} ngia_query_params;
+int ngia_gen_queries(const uint32_t* packed, const uint64_t* word_off, uint32_t n,
+                     const ngia_query_params* p, uint8_t** res, uint64_t** res_off,
+                     uint32_t** res_len, uint8_t** kind);
 
 #ifdef __cplusplus
 }

@@ -6,6 +6,7 @@ module wraps the library with numpy arrays for tests and benchmarks; every
 call goes to the HIP implementation and raises ``NgiaError`` on failure.
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -13,9 +14,14 @@ from . import _lib
 from ._lib import (BANDS, GRAPH_BATCHED, GRAPH_BATCHED_EAGER, GRAPH_PLAIN, NGIA_ERR_ARG, NGIA_ERR_IO,
                    NGIA_ERR_NODEV, NGIA_ERR_STATE, NGIA_OK, BIN_DIR, LIB_PATH)
 
-__all__ = ["NgiaError", "Context", "Database", "PackedDB", "gen_config", "gen_fasta",
-           "gen_packed", "read_db", "leiden", "resolution", "save_result", "fasta_check",
-           "edges_to_array", "BANDS", "GRAPH_PLAIN", "GRAPH_BATCHED", "BIN_DIR", "LIB_PATH"]
+__all__ = ["NgiaError", "Context", "Database", "PackedDB", "Serve", "Queries", "gen_config",
+           "gen_fasta", "gen_packed", "gen_queries", "read_db", "leiden", "resolution",
+           "save_result", "fasta_check", "edges_to_array", "BANDS", "GRAPH_PLAIN",
+           "GRAPH_BATCHED", "BIN_DIR", "LIB_PATH", "ASSIGN_DTYPE", "NO_RESIDENT"]
+
+NO_RESIDENT = _lib.NO_RESIDENT
+ASSIGN_DTYPE = np.dtype([("cluster", np.uint32), ("resident", np.uint32), ("score", np.uint32),
+                         ("candidates", np.uint32)])
 
 
 class NgiaError(RuntimeError):
@@ -63,9 +69,12 @@ class Context:
         h = C.c_void_p()
         _check(L.ngia_open(device, C.byref(h)))
         self.h = h
+        self._children = weakref.WeakSet()  # databases and serving handles on this context
 
     def close(self):
         if self.h:
+            for child in list(self._children):
+                child.close()
             _lib.lib().ngia_close(self.h)
             self.h = None
 
@@ -161,6 +170,7 @@ class Database:
                                          _ptr(self._word_off), _ptr(self._lengths), self.n,
                                          C.byref(h)), ctx.h)
         self.h = h
+        ctx._children.add(self)
 
     def close(self):
         if self.h:
@@ -237,6 +247,106 @@ class Database:
         _check(_lib.lib().ngia_db_fast(self.ctx.h, self.h, identity, bands, _ptr(cl),
                                        C.byref(tot)), self.ctx.h)
         return cl, tot.value
+
+
+class Queries:
+    """Residue strings for serving: query q is res[off[q]:off[q] + length[q]]."""
+
+    def __init__(self, res, off, length, kind=None):
+        self.res = np.ascontiguousarray(res, dtype=np.uint8)
+        self.off = _u64(off)
+        self.length = _u32(length)
+        self.kind = None if kind is None else np.ascontiguousarray(kind, dtype=np.uint8)
+        self.n = len(self.length)
+
+    @classmethod
+    def from_strings(cls, seqs):
+        seqs = [bytes(x) for x in seqs]
+        length = np.array([len(x) for x in seqs], dtype=np.uint32)
+        off = np.zeros(len(seqs), dtype=np.uint64)
+        if len(seqs) > 1:
+            off[1:] = np.cumsum(length[:-1], dtype=np.uint64)
+        res = np.frombuffer(b"".join(seqs) or b"\0", dtype=np.uint8)
+        return cls(res, off, length)
+
+    def seq(self, q):
+        o = int(self.off[q])
+        return self.res[o:o + int(self.length[q])].tobytes()
+
+
+class Serve:
+    """A resident clustered snapshot serving query batches (ngia_serve_*)."""
+
+    def __init__(self, db, cluster_of, identity, new_base, lanes=0, max_queries=0,
+                 max_residues=0, hbm_fraction=0.0):
+        self.ctx = db.ctx
+        self._cluster_of = _u32(cluster_of)
+        assert len(self._cluster_of) == db.n
+        cfg = _lib.ServeConfig(lanes, max_queries, max_residues, hbm_fraction)
+        h = C.c_void_p()
+        _check(_lib.lib().ngia_serve_create(self.ctx.h, db.h, _ptr(self._cluster_of), identity,
+                                            new_base, C.byref(cfg), C.byref(h)), self.ctx.h)
+        self.h = h
+        self.ctx._children.add(self)
+
+    def close(self):
+        if self.h:
+            _lib.lib().ngia_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def assign(self, queries, first_qid=0):
+        """All queries at once (the offline run): structured array of ASSIGN_DTYPE."""
+        out = np.zeros(queries.n, dtype=ASSIGN_DTYPE)
+        st = _lib.ServeStats()
+        _check(_lib.lib().ngia_serve_assign(self.h, _ptr(queries.res), _ptr(queries.off),
+                                            _ptr(queries.length), queries.n, first_qid,
+                                            _ptr(out), C.byref(st)), self.ctx.h)
+        return out, st.as_dict()
+
+    def replay(self, queries, arrival, first, count):
+        """Real-time replay of requests (arrival seconds, query ranges): assignments,
+        per-request latency in seconds and the run's statistics."""
+        nr = len(arrival)
+        reqs = (_lib.Request * max(nr, 1))()
+        for r in range(nr):
+            reqs[r].arrival = float(arrival[r])
+            reqs[r].first = int(first[r])
+            reqs[r].count = int(count[r])
+        out = np.zeros(queries.n, dtype=ASSIGN_DTYPE)
+        lat = np.zeros(max(nr, 1), dtype=np.float64)
+        st = _lib.ServeStats()
+        _check(_lib.lib().ngia_serve_replay(self.h, _ptr(queries.res), _ptr(queries.off),
+                                            _ptr(queries.length), queries.n, reqs, nr, _ptr(out),
+                                            _ptr(lat), C.byref(st)), self.ctx.h)
+        return out, lat[:nr], st.as_dict()
+
+
+def gen_queries(packed, word_off, n_queries, mutated=0.7, sub=(0.0, 0.3), indel=0.01,
+                len_median=300.0, len_sigma=0.5, len_min=50, len_max=1000, seed=1):
+    """Serving queries from a resident packed area (ngia_gen_queries)."""
+    packed, word_off = _u32(packed), _u64(word_off)
+    p = _lib.QueryParams(n_queries, mutated, sub[0], sub[1], indel, len_median, len_sigma,
+                         len_min, len_max, seed)
+    res, off, ln, kind = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    L = _lib.lib()
+    _check(L.ngia_gen_queries(_ptr(packed), _ptr(word_off), len(word_off), C.byref(p),
+                              C.byref(res), C.byref(off), C.byref(ln), C.byref(kind)))
+    try:
+        length = np.ctypeslib.as_array(C.cast(ln, C.POINTER(C.c_uint32)), (max(n_queries, 1),))[:n_queries].copy()
+        offs = np.ctypeslib.as_array(C.cast(off, C.POINTER(C.c_uint64)), (max(n_queries, 1),))[:n_queries].copy()
+        kd = np.ctypeslib.as_array(C.cast(kind, C.POINTER(C.c_uint8)), (max(n_queries, 1),))[:n_queries].copy()
+        total = int(length.astype(np.int64).sum())
+        r = np.ctypeslib.as_array(C.cast(res, C.POINTER(C.c_uint8)), (max(total, 1),))[:max(total, 1)].copy()
+    finally:
+        for h in (res, off, ln, kind):
+            L.ngia_free(h)
+    return Queries(r, offs, length, kd)
 
 
 class PackedDB:

@@ -107,6 +107,39 @@ class GenParams(C.Structure):
     ]
 
 
+class Assignment(C.Structure):
+    _fields_ = [("cluster", C.c_uint32), ("resident", C.c_uint32), ("score", C.c_uint32),
+                ("candidates", C.c_uint32)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("lanes", C.c_uint32), ("max_queries", C.c_uint32),
+                ("max_residues", C.c_uint64), ("hbm_fraction", C.c_double)]
+
+
+class Request(C.Structure):
+    _fields_ = [("arrival", C.c_double), ("first", C.c_uint32), ("count", C.c_uint32)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("queries", C.c_uint64), ("requests", C.c_uint64), ("batches", C.c_uint64),
+                ("pairs", C.c_uint64), ("new_clusters", C.c_uint64), ("lanes", C.c_uint32),
+                ("max_queries", C.c_uint32), ("max_residues", C.c_uint64),
+                ("wall_s", C.c_double), ("device_s", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class QueryParams(C.Structure):
+    _fields_ = [("n_queries", C.c_uint32), ("mutated", C.c_double), ("sub_lo", C.c_double),
+                ("sub_hi", C.c_double), ("indel", C.c_double), ("len_median", C.c_double),
+                ("len_sigma", C.c_double), ("len_min", C.c_uint32), ("len_max", C.c_uint32),
+                ("seed", C.c_uint64)]
+
+
+NO_RESIDENT = 0xFFFFFFFF
+
 P = C.c_void_p
 U32 = C.c_uint32
 U64 = C.c_uint64
@@ -139,6 +172,10 @@ PROTOTYPES = {
     "ngia_db_get_pair_scores": (C.c_int, [P, P, P, P, U64, C.POINTER(U64)]),
     "ngia_db_bin_stats": (C.c_int, [P, P, P, U32, C.POINTER(U32)]),
     "ngia_db_fast": (C.c_int, [P, P, U32, U32, P, C.POINTER(U64)]),
+    "ngia_serve_create": (C.c_int, [P, P, P, U32, U32, C.POINTER(ServeConfig), C.POINTER(P)]),
+    "ngia_serve_destroy": (None, [P]),
+    "ngia_serve_assign": (C.c_int, [P, P, P, P, U32, U64, P, C.POINTER(ServeStats)]),
+    "ngia_serve_replay": (C.c_int, [P, P, P, P, U32, P, U32, P, P, C.POINTER(ServeStats)]),
     "ngia_leiden": (C.c_int, [P, U64, I32, C.c_float, U32, I32, P]),
     "ngia_resolution": (C.c_float, [U32]),
     "ngia_save_result": (C.c_int, [P, U32, C.c_char_p, C.c_char_p]),
@@ -151,6 +188,8 @@ PROTOTYPES = {
     "ngia_gen_config": (C.c_int, [C.c_int, C.c_double, U64, C.POINTER(GenParams)]),
     "ngia_gen_fasta": (C.c_int64, [C.POINTER(GenParams), C.c_char_p]),
     "ngia_gen_packed": (C.c_int, [C.POINTER(GenParams), C.POINTER(PackedDB)]),
+    "ngia_gen_queries": (C.c_int, [P, P, U32, C.POINTER(QueryParams), C.POINTER(P), C.POINTER(P),
+                                   C.POINTER(P), C.POINTER(P)]),
 }
 
 _lib = None

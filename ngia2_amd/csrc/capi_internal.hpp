@@ -1,0 +1,234 @@
+// capi_internal.hpp -- the context and device-resident database behind the
+// C-ABI handles, shared by the graph pipelines (capi.hip) and the serving
+// loop (serve.hip).
+#pragma once
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dev_common.hpp"
+#include "dev_graph.hpp"
+#include "host_common.hpp"
+
+namespace ngia {
+
+constexpr int kSideStreams = 4;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // Grow-only; contents are not preserved.
+  void ensure(size_t b) {
+    if (b <= bytes && p) return;
+    release();
+    size_t want = b < 256 ? 256 : (b + 255) & ~(size_t)255;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw HipError{NGIA_ERR_NOMEM, "hipMalloc(" + std::to_string(want) + " bytes) failed: " +
+                                         hipGetErrorString(e)};
+    }
+    bytes = want;
+  }
+  template <class T>
+  T* as() const { return (T*)p; }
+};
+
+// Device counters of one graph round, cleared by a fill kernel.
+struct Counters {
+  unsigned long long total_cand;  // candidates with duplicates
+  unsigned long long ecount;      // raw edges
+  unsigned long long unique;      // edges after unique
+  unsigned long long bin_tiles[kBins];  // reference tiles of the pairs in each bin
+  uint32_t bin_cnt[kBins];
+  uint32_t bin_off[kBins + 1];
+  uint32_t cursor[kBins];
+  uint32_t work[kBins];
+};
+
+}  // namespace ngia
+
+struct ngia_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t prop{};
+  std::string err;
+  hipEvent_t ev[8] = {};
+  // side streams (default priority) for the concurrent alignment bins
+  hipStream_t side[ngia::kSideStreams] = {};
+  hipEvent_t fork = nullptr, join[ngia::kSideStreams] = {};
+  // start/end of each bin's kernel in the eager batched form
+  hipEvent_t bin_ev[2 * ngia::kBins] = {};
+  uint32_t wave_kmax = 8;  // largest k_align_wave<K> usable on this device
+  // NGIA_BUCKET_WIDE=1 selects the 38-bit-key bucketing that n >= 2^26 uses
+  bool bucket_wide = false;
+  // NGIA_SERIAL_BINS=1 runs the alignment bins one after another on the
+  // context stream (each bin's kernel time is then its time alone)
+  bool serial_bins = false;
+  ngia_timing timing{};    // device time of the last sketch / bucket / graph
+  // live handles made on this context; ngia_close detaches them, so a handle
+  // destroyed after its context only frees its own memory
+  std::vector<ngia_db*> dbs;
+  std::vector<ngia_serve*> serves;
+  ~ngia_ctx() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : join)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : bin_ev)
+      if (e) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    for (auto& q : side)
+      if (q) (void)hipStreamDestroy(q);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct ngia_db {
+  ngia_ctx* ctx = nullptr;
+  uint32_t n = 0;
+  uint64_t n_words = 0;
+  uint32_t max_len = 0;
+  uint32_t rep_bands = 0;
+  bool have_sig = false;
+  ngia::DevBuf packed, word_off, lengths, codes, code_off, sig, rep;
+  // graph workspace
+  ngia::DevBuf flags, pos, sums, jobs, ekeys, escores, ekeys_tmp, escores_tmp, fin_temp;
+  ngia::DevBuf counters, cnt, off, pairs, score, bin_of, list, plain_scratch, plain_jobs, bucket_scratch;
+  uint64_t n_edges = 0, n_pairs = 0;
+  // the captured batched round
+  hipGraphExec_t gexec = nullptr;
+  uint32_t g_identity = 0, g_bands = 0;
+  // per-bin figures of the last batched graph
+  bool have_bins = false;
+  ngia::Counters last{};
+  float bin_ms[ngia::kBins] = {};
+  ~ngia_db() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+  }
+};
+
+// Called by ngia_close for each live serving handle (serve.hip).
+void serve_detach(ngia_serve* sv);
+
+namespace ngia {
+
+inline int fail(ngia_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  set_thread_error(msg);
+  return code;
+}
+
+template <class F>
+inline int guarded(ngia_ctx* ctx, F&& f) {
+  try {
+    return f();
+  } catch (const HipError& e) {
+    return fail(ctx, e.code, e.msg);
+  } catch (const std::bad_alloc&) {
+    return fail(ctx, NGIA_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(ctx, NGIA_ERR_ARG, e.what());
+  }
+}
+
+inline void h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes) NGIA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+}
+inline void d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes) NGIA_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+}
+
+// Validate the packed records before any kernel touches them: every record
+// lies inside the area, its length word matches `lengths` (when given) and is
+// below 0xFFFF (the reference's lines[2048] bound).
+inline void validate_records(const uint32_t* packed, uint64_t n_words, const uint64_t* word_off,
+                      const uint32_t* lengths, uint32_t n, uint32_t* max_len) {
+  uint32_t mx = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    if (word_off[i] >= n_words) throw HipError{NGIA_ERR_ARG, "record offset outside packed area"};
+    const uint32_t L = packed[word_off[i]];
+    if (L >= 0xFFFF) throw HipError{NGIA_ERR_ARG, "record length >= 65535"};
+    if (word_off[i] + rec_words(L) > n_words)
+      throw HipError{NGIA_ERR_ARG, "record extends past packed area"};
+    if (lengths && lengths[i] != L) throw HipError{NGIA_ERR_ARG, "lengths[] disagrees with record"};
+    mx = std::max(mx, L);
+  }
+  *max_len = mx;
+}
+
+inline float event_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  NGIA_HIP(hipEventElapsedTime(&ms, a, b));
+  return ms;
+}
+
+inline uint32_t persistent_blocks(const ngia_ctx* ctx, size_t lds_bytes) {
+  const uint32_t cu = (uint32_t)std::max(1, ctx->prop.multiProcessorCount);
+  uint32_t per_cu = lds_bytes ? (uint32_t)std::min<size_t>(16, (160 * 1024) / lds_bytes) : 16;
+  if (per_cu < 1) per_cu = 1;
+  return cu * per_cu;
+}
+
+inline void db_upload(ngia_ctx* ctx, ngia_db* db, const uint32_t* packed, uint64_t n_words,
+               const uint64_t* word_off, const uint32_t* lengths, uint32_t n) {
+  std::vector<uint32_t> len_v;
+  if (!lengths) {
+    len_v.resize(n);
+    for (uint32_t i = 0; i < n; i++) len_v[i] = packed[word_off[i]];
+    lengths = len_v.data();
+  }
+  validate_records(packed, n_words, word_off, lengths, n, &db->max_len);
+  db->n = n;
+  db->n_words = n_words;
+  hipStream_t s = ctx->stream;
+  db->packed.ensure(4 * std::max<uint64_t>(n_words, 1));
+  db->word_off.ensure(8ull * std::max<uint32_t>(n, 1));
+  db->lengths.ensure(4ull * std::max<uint32_t>(n, 1));
+  h2d(db->packed.p, packed, 4 * n_words, s);
+  h2d(db->word_off.p, word_off, 8ull * n, s);
+  h2d(db->lengths.p, lengths, 4ull * n, s);
+  // byte codes: 32 bytes per 32-residue block, sequence i at code_off[i]
+  std::vector<uint64_t> coff(std::max<uint32_t>(n, 1));
+  uint64_t acc = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    coff[i] = acc;
+    acc += 32ull * ((lengths[i] + 31) / 32);
+  }
+  db->code_off.ensure(8ull * std::max<uint32_t>(n, 1));
+  db->codes.ensure(acc + 64);
+  h2d(db->code_off.p, coff.data(), 8ull * n, s);
+  launch_codes(db->packed.as<uint32_t>(), db->word_off.as<uint64_t>(), db->code_off.as<uint64_t>(),
+               n, db->codes.as<uint8_t>(), s);
+  NGIA_HIP(hipStreamSynchronize(s));
+  db->have_sig = false;
+  db->rep_bands = 0;
+  if (db->gexec) {
+    (void)hipGraphExecDestroy(db->gexec);
+    db->gexec = nullptr;
+  }
+}
+
+inline AlignArgs align_args(const ngia_db* db, float thr) {
+  AlignArgs a;
+  a.packed = db->packed.as<uint32_t>();
+  a.word_off = db->word_off.as<uint64_t>();
+  a.codes = db->codes.as<uint8_t>();
+  a.code_off = db->code_off.as<uint64_t>();
+  a.thr = thr;
+  return a;
+}
+
+}  // namespace ngia

@@ -45,5 +45,10 @@ inline uint32_t rec_words(uint32_t L) { return 1 + (L + 31) / 32 * 4; }
 uint64_t assign_offsets(std::vector<SeqIndex>* idx);
 // Pack one residue string into a record (makedb.hpp:222-231).
 void pack_record(const uint8_t* res, uint32_t L, uint32_t* rec);
+// Records of residue strings first..first+count-1 (string q at res + res_off[q],
+// res_len[q] residues), record j at dst + word_pos[j] - word_base; OpenMP.
+void pack_records(const uint8_t* res, const uint64_t* res_off, const uint32_t* res_len,
+                  uint32_t first, uint32_t count, const uint64_t* word_pos, uint64_t word_base,
+                  uint32_t* dst);
 
 }  // namespace ngia

@@ -54,6 +54,16 @@ void pack_record(const uint8_t* res, uint32_t L, uint32_t* rec) {
   }
 }
 
+void pack_records(const uint8_t* res, const uint64_t* res_off, const uint32_t* res_len,
+                  uint32_t first, uint32_t count, const uint64_t* word_pos, uint64_t word_base,
+                  uint32_t* dst) {
+  #pragma omp parallel for schedule(dynamic, 64) if (count > 512)
+  for (int64_t j = 0; j < (int64_t)count; j++) {
+    const uint32_t q = first + (uint32_t)j;
+    pack_record(res + res_off[q], res_len[q], dst + (word_pos[j] - word_base));
+  }
+}
+
 namespace {
 struct MappedFile {
   const uint8_t* data = nullptr;

@@ -225,3 +225,63 @@ extern "C" int ngia_gen_packed(const ngia_gen_params* p, ngia_packed_db* out) {
   }
   return NGIA_OK;
 }
+
+extern "C" int ngia_gen_queries(const uint32_t* packed, const uint64_t* word_off, uint32_t n,
+                                const ngia_query_params* p, uint8_t** res, uint64_t** res_off,
+                                uint32_t** res_len, uint8_t** kind) {
+  if (!p || !res || !res_off || !res_len || !kind || (n && (!packed || !word_off)))
+    return NGIA_ERR_ARG;
+  if (p->mutated > 0 && n == 0) return NGIA_ERR_ARG;
+  const uint32_t nq = p->n_queries;
+  // A letter for each code (the first letter of makedb.hpp:164-169 mapping to it).
+  static const char kLetter[16] = {'X', 'A', 'C', 'D', 'E', 'F', 'G', 'H',
+                                   'I', 'K', 'L', 'N', 'P', 'R', 'S', 'T'};
+  std::vector<std::string> q(nq);
+  std::vector<uint8_t> kd(nq);
+  #pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t k = 0; k < (int64_t)nq; k++) {
+    Rng r(p->seed * 0xd1342543de82ef95ULL + (uint64_t)k + 7);
+    std::string& s = q[k];
+    if (r.uniform() < p->mutated) {
+      kd[k] = 1;
+      const uint32_t* rec = packed + word_off[r.below(n)];
+      const uint32_t L = rec[0];
+      const double sub = p->sub_lo + (p->sub_hi - p->sub_lo) * r.uniform();
+      s.reserve(L + L / 8 + 8);
+      for (uint32_t j = 0; j < L; j++) {
+        const uint32_t* pl = rec + 1 + 4 * (j / 32);
+        const uint32_t b = j % 32;
+        const uint32_t code = ((pl[0] >> b) & 1u) | (((pl[1] >> b) & 1u) << 1) |
+                              (((pl[2] >> b) & 1u) << 2) | (((pl[3] >> b) & 1u) << 3);
+        if (r.uniform() < p->indel) {
+          if (r.uniform() < 0.5) continue;   // deletion
+          s.push_back(kAmino[r.below(20)]);  // insertion before j
+        }
+        s.push_back(r.uniform() < sub ? kAmino[r.below(20)] : kLetter[code]);
+      }
+    } else {
+      kd[k] = 0;
+      double L = std::floor(p->len_median * std::exp(p->len_sigma * r.normal()) + 0.5);
+      L = std::min<double>(std::max<double>(L, p->len_min), p->len_max);
+      s.resize((size_t)L);
+      for (auto& c : s) c = kAmino[r.below(20)];
+    }
+    if (s.size() >= 0xFFFF) s.resize(0xFFFE);
+  }
+  uint64_t total = 0;
+  for (const auto& s : q) total += s.size();
+  *res = (uint8_t*)std::malloc(std::max<uint64_t>(total, 1));
+  *res_off = (uint64_t*)std::malloc(8ull * std::max<uint32_t>(nq, 1));
+  *res_len = (uint32_t*)std::malloc(4ull * std::max<uint32_t>(nq, 1));
+  *kind = (uint8_t*)std::malloc(std::max<uint32_t>(nq, 1));
+  if (!*res || !*res_off || !*res_len || !*kind) return NGIA_ERR_NOMEM;
+  uint64_t off = 0;
+  for (uint32_t k = 0; k < nq; k++) {
+    (*res_off)[k] = off;
+    (*res_len)[k] = (uint32_t)q[k].size();
+    (*kind)[k] = kd[k];
+    std::memcpy(*res + off, q[k].data(), q[k].size());
+    off += q[k].size();
+  }
+  return NGIA_OK;
+}

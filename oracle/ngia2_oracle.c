@@ -821,3 +821,73 @@ int orc_save_result(const uint32_t* cluster, uint32_t n, const char* packed_path
   free(cnt); free(start); free(fill); free(members); free(db);
   return 0;
 }
+
+/* ------------------------------------------------------------- serving */
+static int cmp_u64(const void* a, const void* b) {
+  const uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+  return x < y ? -1 : x > y;
+}
+
+void orc_serve_assign(const uint32_t* packed, const uint64_t* word_off, uint32_t n,
+                      const uint32_t* sig, const uint32_t* cluster_of, const uint32_t* qpacked,
+                      const uint64_t* qword_off, uint32_t nq, uint32_t identity,
+                      uint32_t new_base, uint64_t first_qid, uint32_t* out, int nthreads) {
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+  const float thr = (float)identity / 100.0f;
+  /* per band: (sig << 32 | i) sorted, so the first entry of a signature is its lowest i */
+  uint64_t* sorted = (uint64_t*)malloc(sizeof(uint64_t) * ORC_BANDS * (n ? n : 1));
+  for (uint32_t b = 0; b < ORC_BANDS; b++) {
+    uint64_t* row = sorted + (size_t)b * n;
+    for (uint32_t i = 0; i < n; i++) row[i] = ((uint64_t)sig[(size_t)i * ORC_BANDS + b] << 32) | i;
+    qsort(row, n, sizeof(uint64_t), cmp_u64);
+  }
+  #pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+  for (int64_t q = 0; q < (int64_t)nq; q++) {
+    const uint32_t* qrec = qpacked + qword_off[q];
+    const uint32_t Lq = qrec[0];
+    uint32_t qs[ORC_BANDS], heads[ORC_BANDS];
+    orc_sketch_record(qrec, qs);
+    uint32_t nh = 0;
+    for (uint32_t b = 0; b < ORC_BANDS; b++) {
+      const uint64_t* row = sorted + (size_t)b * n;
+      const uint64_t key = (uint64_t)qs[b] << 32;
+      uint32_t lo = 0, hi = n;  /* first entry >= key */
+      while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (row[mid] < key) lo = mid + 1; else hi = mid;
+      }
+      if (lo == n || (uint32_t)(row[lo] >> 32) != qs[b]) continue;
+      const uint32_t h = (uint32_t)row[lo];
+      int seen = 0;
+      for (uint32_t k = 0; k < nh; k++) seen |= heads[k] == h;
+      if (!seen) heads[nh++] = h;
+    }
+    uint32_t best_score = 0, best_res = 0xFFFFFFFFu, cand = 0;
+    for (uint32_t k = 0; k < nh; k++) {
+      const uint32_t h = heads[k];
+      const uint32_t* hrec = packed + word_off[h];
+      const uint32_t Lr = hrec[0];
+      const uint32_t* rep = Lr >= Lq ? hrec : qrec;
+      const uint32_t* tar = Lr >= Lq ? qrec : hrec;
+      if ((float)tar[0] < (float)rep[0] * thr) continue;
+      cand++;
+      const uint32_t sc = orc_align_record(rep, tar, thr, 1);
+      if (sc > best_score || (sc == best_score && sc > 0 && h < best_res)) {
+        best_score = sc;
+        best_res = h;
+      }
+    }
+    uint32_t* o = out + 4 * (size_t)q;
+    if (best_score == 0) {
+      o[0] = new_base + (uint32_t)(first_qid + (uint64_t)q);
+      o[1] = 0xFFFFFFFFu;
+      o[2] = 0;
+    } else {
+      o[0] = cluster_of[best_res];
+      o[1] = best_res;
+      o[2] = best_score;
+    }
+    o[3] = cand;
+  }
+  free(sorted);
+}

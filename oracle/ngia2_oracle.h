@@ -31,6 +31,9 @@
  *                              same edge lists; mt19937_64 known-answer value
  *   orc_save_result            reference SaveResult build (oracle/_ref)
  *   orc_fast_cluster           brute restatement in tests (parent chains)
+ *   orc_serve_assign           heads equal the groupSignatures chain heads of
+ *                              resident queries; scores equal orc_align_record
+ *                              of the chosen pair; brute force in tests
  * Parity unpinned: nvcc's choice of FMA contraction for the band edges in the
  * reference kernel (buildGraph.hpp:208-210) cannot be observed without nvcc;
  * both variants are provided (fused = nvcc -fmad=true default).
@@ -86,6 +89,18 @@ int orc_leiden(const orc_edge* edges, uint64_t m, int32_t n, float resolution,
 int orc_fast_cluster(const uint32_t* packed, const uint64_t* word_off, const uint32_t* len,
                      uint32_t n, const uint32_t* rep_rows, uint32_t bands, uint32_t identity,
                      int fused, int nthreads, uint32_t* cluster, uint64_t* total_pairs);
+/* Serving assignment (no reference counterpart; the rule is stated with
+ * ngia_serve_create in include/ngia2.h and built from the pieces above):
+ * per query, sketch (makedb.hpp:224-247); per band the lowest resident index
+ * with the query's signature (the bucket head of makedb.hpp:340-347); the
+ * distinct heads as pairs with the longer sequence as rep (ties: resident),
+ * the isJob length test (buildGraph.hpp:262-266) and the kernel_dynamic score;
+ * best = highest score, then lowest resident.  out[4q..4q+3] = cluster,
+ * resident (0xFFFFFFFF for a new cluster), score, candidates. */
+void orc_serve_assign(const uint32_t* packed, const uint64_t* word_off, uint32_t n,
+                      const uint32_t* sig, const uint32_t* cluster_of, const uint32_t* qpacked,
+                      const uint64_t* qword_off, uint32_t nq, uint32_t identity,
+                      uint32_t new_base, uint64_t first_qid, uint32_t* out, int nthreads);
 /* makedb.hpp:94-364 end to end; returns 0 ok, 1 runtime failure, 2 check failure */
 int orc_makedb(const char* fasta, const char* packed, int nthreads);
 /* saveResult.hpp:31-83 */

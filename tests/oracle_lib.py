@@ -56,6 +56,8 @@ def orc():
         L.orc_fast_cluster.argtypes = [P, P, P, U32, P, U32, U32, C.c_int, C.c_int, P,
                                        C.POINTER(U64)]
         L.orc_makedb.restype = C.c_int
+        L.orc_serve_assign.argtypes = [P, P, U32, P, P, P, P, U32, U32, U32, U64, P, C.c_int]
+        L.orc_serve_assign.restype = None
         L.orc_makedb.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
         L.orc_save_result.restype = C.c_int
         L.orc_save_result.argtypes = [P, U32, C.c_char_p, C.c_char_p]
@@ -208,6 +210,32 @@ def fast_cluster(packed, word_off, lengths, rep_rows, bands, identity, fused=1, 
     orc().orc_fast_cluster(_p(packed), _p(word_off), _p(lengths), n, _p(rep_rows), bands,
                            identity, fused, threads, _p(out), C.byref(tot))
     return out, tot.value
+
+
+def pack_many(seqs):
+    """Packed records of residue strings: (packed area, word offsets)."""
+    recs = [pack(x) for x in seqs]
+    off = np.zeros(len(recs), dtype=np.uint64)
+    if len(recs) > 1:
+        off[1:] = np.cumsum([len(r) for r in recs[:-1]])
+    packed = np.concatenate(recs).astype(np.uint32) if recs else np.zeros(1, np.uint32)
+    return packed, off
+
+
+def serve_assign(packed, word_off, sig, cluster_of, qseqs, identity, new_base, first_qid=0,
+                 threads=0):
+    """Serving assignment of residue strings qseqs: (nq, 4) uint32 rows of
+    (cluster, resident, score, candidates)."""
+    packed = np.ascontiguousarray(packed, dtype=np.uint32)
+    word_off = np.ascontiguousarray(word_off, dtype=np.uint64)
+    sig = np.ascontiguousarray(sig, dtype=np.uint32)
+    cluster_of = np.ascontiguousarray(cluster_of, dtype=np.uint32)
+    qp, qo = pack_many(qseqs)
+    out = np.zeros((len(qseqs), 4), dtype=np.uint32)
+    orc().orc_serve_assign(_p(packed), _p(word_off), len(word_off), _p(sig), _p(cluster_of),
+                           _p(qp), _p(qo), len(qseqs), identity, new_base, first_qid, _p(out),
+                           threads)
+    return out
 
 
 def makedb(fasta, packed, threads=0):
