@@ -154,6 +154,87 @@ def measure(ng, np, ctx, info, d, identity, steps, warmup, plain_steps, label):
     return out
 
 
+def serving_bench(ng, np, ctx, d, identity, n_queries, seed, recluster_ms_per_seq):
+    """Serving over the resident set: cluster it offline (batched graph + host
+    Leiden), then assign a stream of query batches (70% mutated residents at
+    0-30% divergence, 30% novel; batch sizes log-uniform over 1..4096) with
+    Poisson arrivals at several offered loads."""
+    t0 = time.perf_counter()
+    db = ng.Database(ctx, d.packed, d.word_off, d.lengths)
+    db.sketch()
+    db.bucket()
+    db.graph(identity, 64, ng.GRAPH_BATCHED)
+    edges = db.edges()
+    t_graph = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    part = ng.leiden(edges, d.n, ng.resolution(identity))
+    t_leiden = time.perf_counter() - t0
+    new_base = int(part.max()) + 1
+    log("serving: resident clustering %.2f s (graph) + %.2f s (Leiden), %d clusters"
+        % (t_graph, t_leiden, new_base))
+    q = ng.gen_queries(d.packed, d.word_off, n_queries, mutated=0.7, sub=(0.0, 0.3), indel=0.01,
+                       len_median=300.0, len_sigma=0.5, len_min=30, len_max=5000, seed=seed)
+    rng = np.random.default_rng(seed)
+    sizes = []
+    left = q.n
+    while left > 0:
+        k = int(min(max(np.floor(np.exp(rng.uniform(0.0, np.log(4097.0)))), 1), 4096, left))
+        sizes.append(k)
+        left -= k
+    count = np.array(sizes, dtype=np.int64)
+    first = np.concatenate([[0], np.cumsum(count)[:-1]])
+    sv = ng.Serve(db, part, identity, new_base)
+    sv.assign(q)  # warm-up
+    t0 = time.perf_counter()
+    offline, st_off = sv.assign(q)
+    t_off = time.perf_counter() - t0
+    sat_qps = q.n / t_off
+    log("serving: offline assignment of %d queries in %.1f ms (%.0f queries/s)"
+        % (q.n, 1e3 * t_off, sat_qps))
+    loads = []
+    identical = True
+    for rho in (0.25, 0.5, 0.8):
+        rate = rho * sat_qps / float(count.mean())  # requests per second
+        arrival = np.cumsum(rng.exponential(1.0 / rate, len(count)))
+        arrival -= arrival[0]
+        got, lat, st = sv.replay(q, arrival, first, count)
+        same = bool(np.array_equal(got, offline))
+        identical &= same
+        loads.append({"offered_load": rho, "request_rate_per_s": round(rate, 1),
+                      "queries_per_s": round(q.n / st["wall_s"], 1),
+                      "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+                      "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+                      "max_ms": round(1e3 * float(lat.max()), 3),
+                      "batches": st["batches"], "device_s": round(st["device_s"], 4),
+                      "wall_s": round(st["wall_s"], 4), "identical_to_offline": same})
+        log("serving: load %.2f: p50 %.2f ms, p99 %.2f ms, %d batches"
+            % (rho, loads[-1]["p50_ms"], loads[-1]["p99_ms"], st["batches"]))
+    sv.close()
+    db.close()
+    kinds = q.kind.astype(bool)
+    out = {"workload": "resident: the benchmark set clustered at -i %d (batched graph + Leiden, "
+                       "%d clusters); %d queries, 70%% mutated residents (0-30%% substitutions, 1%% "
+                       "indels), 30%% novel (log-normal lengths, median 300); %d requests, "
+                       "log-uniform sizes 1-4096, Poisson arrivals" % (identity, new_base, q.n,
+                                                                         len(count)),
+           "residents": d.n, "queries": q.n, "requests": len(count),
+           "lanes": st_off["lanes"], "max_queries_per_batch": st_off["max_queries"],
+           "max_residues_per_batch": st_off["max_residues"],
+           "saturated_queries_per_s": round(sat_qps, 1),
+           "offline_assign_ms": round(1e3 * t_off, 3),
+           "loads": loads, "assignments_identical_to_offline": identical,
+           "new_clusters": int((offline["resident"] == ng.NO_RESIDENT).sum()),
+           "mutated_joined": round(float((offline["resident"][kinds] != ng.NO_RESIDENT).mean()), 4)
+           if kinds.any() else None,
+           "resident_clustering_s": {"graph": round(t_graph, 3), "leiden": round(t_leiden, 3)},
+           "offline_recluster": {
+               "sequences": d.n + q.n,
+               "ms": round(recluster_ms_per_seq * (d.n + q.n), 3),
+               "basis": "the batched hot path's measured time per sequence on the resident set, "
+                        "times residents + queries"}}
+    return out
+
+
 def align_rooflines(m, peak):
     rows = []
     for b in m["bins"]:
@@ -201,6 +282,8 @@ def main():
     ap.add_argument("--cpu-scale", type=float, default=0.1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serve-queries", type=int, default=300000,
+                    help="queries of the serving stream over the resident set (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="bytes of one device-to-device copy run first (counter calibration)")
     args = ap.parse_args()
@@ -288,6 +371,15 @@ def main():
     if "plain" in m:
         out["plain_form"] = m["plain"]
         out["batched_vs_plain"] = round(m["value"] / m["plain"]["value"], 3)
+
+    if args.serve_queries:
+        sv = serving_bench(ng, np, ctx, d, args.identity, args.serve_queries, args.seed + 100,
+                           m["ms_per_step"] / d.n)
+        out["serving"] = sv
+        mid = sv["loads"][1]
+        out["latency_ms"] = {"p50": mid["p50_ms"], "p99": mid["p99_ms"],
+                             "offered_load": mid["offered_load"],
+                             "queries_per_s": mid["queries_per_s"]}
 
     if args.also_config:
         p2 = ng.gen_config(args.also_config, args.scale, args.seed)
