@@ -284,6 +284,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--serve-queries", type=int, default=300000,
                     help="queries of the serving stream over the resident set (0: skip)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the window-bin autotuner")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="bytes of one device-to-device copy run first (counter calibration)")
     args = ap.parse_args()
@@ -302,6 +303,16 @@ def main():
     ctx = ng.Context(0)
     info = ctx.info()
     peak = valu_peak_tops(info)
+    tune = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        pts = ctx.autotune(args.identity)
+        route = ctx.tuning()
+        tune = {"identity": args.identity, "seconds": round(time.perf_counter() - t0, 3),
+                "route_below": {str(W): route[W] for W in range(2, 17) if route[W]},
+                "points": [[p["width"], p["pairs"], round(p["window_ms"], 4), round(p["wave_ms"], 4)]
+                           for p in pts]}
+        log("autotune: %.2f s, route_below %s" % (tune["seconds"], tune["route_below"]))
     traffic = load_traffic()
 
     t0 = time.perf_counter()
@@ -366,6 +377,7 @@ def main():
            "align_phase": dict(phase, peak=round(peak, 3), unit="TOP/s int32",
                                eager_ms_per_step=round(m["eager_ms_per_step"], 3)),
            "align_kernels": align_rows[:8],
+           "autotune": tune,
            "device": {"name": info["name"], "arch": info["arch"], "cu": info["cu_count"],
                       "clock_mhz": info["clock_khz"] / 1e3}}
     if "plain" in m:

@@ -173,6 +173,35 @@ typedef struct {
 int ngia_db_bin_stats(ngia_ctx* ctx, ngia_db* db, ngia_bin_stat* out, uint32_t cap,
                       uint32_t* n_bins);
 
+/* ------------------------------------------- autotuner */
+/* No reference counterpart.  The batched form aligns a window bin (band
+ * window W = 2..16 words) with one thread per pair; a bin with few pairs
+ * finishes sooner with one wavefront per pair (k_align_wave<1>), because a
+ * lone thread runs its pair's T x W word-characters serially.  Where the two
+ * cross depends on W, the identity (pair length) and the device. */
+typedef struct {
+  /* window bin W goes to the wavefront kernel when it holds fewer than
+   * route_below[W] pairs; 0 = never (the state of a fresh context) */
+  uint32_t route_below[17];
+} ngia_tuning;
+int ngia_get_tuning(ngia_ctx* ctx, ngia_tuning* out);
+int ngia_set_tuning(ngia_ctx* ctx, const ngia_tuning* t);
+typedef struct {
+  uint32_t width;    /* band window W */
+  uint32_t length;   /* rep length of the synthetic pairs (targets are up to 64 shorter) */
+  uint32_t pairs;    /* pairs in the timed bin */
+  double window_ms;  /* k_align_win<W>, device time (HIP events) */
+  double wave_ms;    /* k_align_wave<1> on the same pairs */
+} ngia_tune_point;
+/* Time both kernels on synthetic pairs of every window width W at `identity`
+ * (random reps with targets of nearly the same length, chosen so that the
+ * band window is exactly W),
+ * at bins of 16..4096 pairs, and install the crossovers in the context's
+ * tuning: route_below[W] = the smallest timed bin size at which the window
+ * kernel is no slower.  points (may be NULL) receive the measurements. */
+int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* points, uint32_t cap,
+                  uint32_t* n_points);
+
 /* ------------------------------------------- fast mode (no graph, no Leiden) */
 /* FastClustering::cluster (fastClustering.hpp:249-352): per band, align the
  * still-unclustered targets, parent[tar] = rep on a pass; cluster[i] is the

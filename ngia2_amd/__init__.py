@@ -78,6 +78,28 @@ class Context:
             _lib.lib().ngia_close(self.h)
             self.h = None
 
+    def tuning(self):
+        """route_below[W] of the window bins (ngia_get_tuning)."""
+        t = _lib.Tuning()
+        _check(_lib.lib().ngia_get_tuning(self.h, C.byref(t)), self.h)
+        return list(t.route_below)
+
+    def set_tuning(self, route_below):
+        t = _lib.Tuning()
+        for W, v in enumerate(route_below):
+            t.route_below[W] = int(v)
+        _check(_lib.lib().ngia_set_tuning(self.h, C.byref(t)), self.h)
+
+    def autotune(self, identity):
+        """Time the window and wavefront kernels per window width and install the
+        crossovers (ngia_autotune); returns the measured points."""
+        cap = 256
+        pts = (_lib.TunePoint * cap)()
+        n = C.c_uint32()
+        _check(_lib.lib().ngia_autotune(self.h, identity, pts, cap, C.byref(n)), self.h)
+        return [{"width": p.width, "length": p.length, "pairs": p.pairs,
+                 "window_ms": p.window_ms, "wave_ms": p.wave_ms} for p in pts[:min(n.value, cap)]]
+
     def __enter__(self):
         return self
 

@@ -53,6 +53,15 @@ class BinStat(C.Structure):
 BIN_KINDS = {0: "zero", 1: "window", 2: "wave", 3: "plain"}
 
 
+class Tuning(C.Structure):
+    _fields_ = [("route_below", C.c_uint32 * 17)]
+
+
+class TunePoint(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("length", C.c_uint32), ("pairs", C.c_uint32),
+                ("window_ms", C.c_double), ("wave_ms", C.c_double)]
+
+
 class Timing(C.Structure):
     _fields_ = [("sketch_ms", C.c_double), ("bucket_ms", C.c_double), ("graph_ms", C.c_double),
                 ("finalize_ms", C.c_double), ("align_ms", C.c_double)]
@@ -171,6 +180,9 @@ PROTOTYPES = {
     "ngia_db_get_edges": (C.c_int, [P, P, P, U64, C.POINTER(U64)]),
     "ngia_db_get_pair_scores": (C.c_int, [P, P, P, P, U64, C.POINTER(U64)]),
     "ngia_db_bin_stats": (C.c_int, [P, P, P, U32, C.POINTER(U32)]),
+    "ngia_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+    "ngia_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+    "ngia_autotune": (C.c_int, [P, U32, P, U32, C.POINTER(U32)]),
     "ngia_db_fast": (C.c_int, [P, P, U32, U32, P, C.POINTER(U64)]),
     "ngia_serve_create": (C.c_int, [P, P, P, U32, U32, C.POINTER(ServeConfig), C.POINTER(P)]),
     "ngia_serve_destroy": (None, [P]),

@@ -139,7 +139,7 @@ void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ban
   const uint32_t grid = persistent_blocks(ctx, 0) * 4;
   bin_pairs(db->pairs.as<uint2>(), npairs, len, thr, db->bin_of.as<uint32_t>(), dc->bin_cnt,
             dc->bin_off, dc->cursor, db->list.as<uint32_t>(), dc->bin_tiles, db->score.as<uint32_t>(),
-            ctx->wave_kmax, grid, s);
+            ctx->wave_kmax, dc->remap, ctx->route, grid, s);
   const AlignArgs aa = align_args(db, thr);
   // The bins run concurrently, forked onto the side streams and joined back.
   // The pairs of the wide wave bins are the longest, so the bins start

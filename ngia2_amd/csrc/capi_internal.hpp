@@ -4,6 +4,7 @@
 #pragma once
 
 #include <algorithm>
+#include <functional>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -56,6 +57,7 @@ struct Counters {
   uint32_t bin_off[kBins + 1];
   uint32_t cursor[kBins];
   uint32_t work[kBins];
+  uint32_t remap[kBins];
 };
 
 }  // namespace ngia
@@ -72,6 +74,7 @@ struct ngia_ctx {
   // start/end of each bin's kernel in the eager batched form
   hipEvent_t bin_ev[2 * ngia::kBins] = {};
   uint32_t wave_kmax = 8;  // largest k_align_wave<K> usable on this device
+  ngia::RouteTable route{};  // small window bins to the wavefront kernel (autotuner)
   // NGIA_BUCKET_WIDE=1 selects the 38-bit-key bucketing that n >= 2^26 uses
   bool bucket_wide = false;
   // NGIA_SERIAL_BINS=1 runs the alignment bins one after another on the

@@ -338,10 +338,23 @@ __global__ void k_bin_count(const uint2* __restrict__ pairs, const uint64_t* __r
   }
 }
 
-// bin_off[b] = exclusive prefix of bin_cnt; also resets the scatter cursors.
-__global__ void k_bin_offsets(const uint32_t* __restrict__ bin_cnt, uint32_t* __restrict__ bin_off,
-                              uint32_t* __restrict__ cursor) {
+// Route small window bins to the wavefront kernel (remap[b] = the bin that
+// aligns bin b's pairs, counts and tiles moved along), then bin_off[b] =
+// exclusive prefix of bin_cnt; also resets the scatter cursors.
+__global__ void k_bin_offsets(uint32_t* __restrict__ bin_cnt, uint32_t* __restrict__ bin_off,
+                              uint32_t* __restrict__ cursor, uint32_t* __restrict__ remap,
+                              unsigned long long* __restrict__ bin_tiles, RouteTable route) {
   if (threadIdx.x == 0) {
+    for (int b = 0; b < kBins; b++) remap[b] = (uint32_t)b;
+    for (int W = 2; W <= kWinMax; W++) {
+      if (bin_cnt[W] && bin_cnt[W] < route.route_below[W]) {
+        remap[W] = kBinWave0;  // k_align_wave<1>: W <= 64
+        bin_cnt[kBinWave0] += bin_cnt[W];
+        bin_tiles[kBinWave0] += bin_tiles[W];
+        bin_cnt[W] = 0;
+        bin_tiles[W] = 0;
+      }
+    }
     uint32_t s = 0;
     for (int b = 0; b < kBins; b++) {
       bin_off[b] = s;
@@ -357,6 +370,7 @@ __global__ void k_bin_offsets(const uint32_t* __restrict__ bin_cnt, uint32_t* __
 // (tile, bin), then written.
 __global__ __launch_bounds__(256) void k_bin_scatter(const uint64_t* __restrict__ npairs,
                                                      const uint32_t* __restrict__ bin_of,
+                                                     const uint32_t* __restrict__ remap,
                                                      const uint32_t* __restrict__ bin_off,
                                                      uint32_t* __restrict__ cursor,
                                                      uint32_t* __restrict__ list) {
@@ -369,7 +383,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const uint64_t* __restrict_
     #pragma unroll
     for (int k = 0; k < 16; k++) {
       const uint64_t p = base + (uint64_t)k * 256 + threadIdx.x;
-      b[k] = p < P ? bin_of[p] : 0u;
+      b[k] = p < P ? remap[bin_of[p]] : 0u;
       pos[k] = p < P ? atomicAdd(&lcnt[b[k]], 1u) : 0u;
     }
     __syncthreads();
@@ -388,12 +402,13 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const uint64_t* __restrict_
 void bin_pairs(const uint2* pairs, const uint64_t* npairs, const uint32_t* len, float thr,
                uint32_t* bin_of, uint32_t* bin_cnt, uint32_t* bin_off, uint32_t* cursor,
                uint32_t* list, unsigned long long* bin_tiles, uint32_t* score, uint32_t kmax,
-               uint32_t grid, hipStream_t s) {
+               uint32_t* remap, const RouteTable& route, uint32_t grid, hipStream_t s) {
   hipLaunchKernelGGL(k_bin_count, dim3(grid), dim3(256), 0, s, pairs, npairs, len, thr, kmax, bin_of,
                      bin_cnt, bin_tiles, score);
-  hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, bin_cnt, bin_off, cursor);
-  hipLaunchKernelGGL(k_bin_scatter, dim3(grid), dim3(256), 0, s, npairs, bin_of, bin_off, cursor,
-                     list);
+  hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, bin_cnt, bin_off, cursor, remap,
+                     bin_tiles, route);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(grid), dim3(256), 0, s, npairs, bin_of, remap, bin_off,
+                     cursor, list);
   NGIA_HIP(hipGetLastError());
 }
 

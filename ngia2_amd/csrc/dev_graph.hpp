@@ -12,6 +12,13 @@ constexpr int kBinWave0 = kWinMax + 1;  // kWinMax+1 .. +6 for K = 1..32
 constexpr int kBinPlain = kBinWave0 + 6;
 constexpr int kBins = kBinPlain + 1;
 
+// Routing of small window bins, set by the autotuner: window bin W with
+// fewer than route_below[W] pairs is aligned by k_align_wave<1> instead (a
+// wavefront per pair finishes a few pairs sooner than one thread per pair).
+struct RouteTable {
+  uint32_t route_below[kWinMax + 1];
+};
+
 void fill_u32(uint32_t* p, uint32_t v, uint64_t count, hipStream_t s);
 uint64_t scan_scratch_words(uint32_t n);
 // out[0..n] = exclusive prefix sums of in[0..n), out[n] = total (n >= 1).
@@ -34,7 +41,7 @@ void pair_emit(const uint32_t* rep, const uint32_t* len, uint32_t n, uint32_t ba
 void bin_pairs(const uint2* pairs, const uint64_t* npairs, const uint32_t* len, float thr,
                uint32_t* bin_of, uint32_t* bin_cnt, uint32_t* bin_off, uint32_t* cursor,
                uint32_t* list, unsigned long long* bin_tiles, uint32_t* score, uint32_t kmax,
-               uint32_t grid, hipStream_t s);
+               uint32_t* remap, const RouteTable& route, uint32_t grid, hipStream_t s);
 void plain_bin_jobs(const uint2* pairs, const uint32_t* list, const uint32_t* bin_off,
                     const uint32_t* bin_cnt, uint32_t* jobs, hipStream_t s);
 void plain_bin_scores(const uint32_t* list, const uint32_t* bin_off, const uint32_t* bin_cnt,

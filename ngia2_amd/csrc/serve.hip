@@ -349,7 +349,7 @@ void launch_batch(ngia_serve* sv, uint32_t k, const uint8_t* res, const uint64_t
   const uint32_t grid = persistent_blocks(sv->ctx, 0);
   bin_pairs(ln.pairs.as<uint2>(), np, sv->lengths.as<uint32_t>(), sv->thr, ln.bin_of.as<uint32_t>(),
             dc->bin_cnt, dc->bin_off, dc->cursor, ln.list.as<uint32_t>(), dc->bin_tiles,
-            ln.score.as<uint32_t>(), sv->ctx->wave_kmax, grid, s);
+            ln.score.as<uint32_t>(), sv->ctx->wave_kmax, dc->remap, sv->ctx->route, grid, s);
   AlignArgs aa;
   aa.packed = sv->packed.as<uint32_t>();
   aa.word_off = sv->word_off.as<uint64_t>();

@@ -295,3 +295,49 @@ def test_eager_batched_and_bin_accounting(ctx, longdb, identity):
     for b in used:
         if b["kind"] in ("window", "wave", "plain"):
             assert b["ms"] > 0, b
+
+
+def test_routed_window_bins_score_alike(ctx, c1db):
+    """Window bins routed to the wavefront kernel (any bin size) score every pair
+    as the oracle does, and the bin accounting follows the pairs."""
+    _, _, d = c1db
+    say("routed window bins")
+    db = ng.Database(ctx, d["packed"], d["word_off"], d["lengths"])
+    db.set_reps(d["rep"], 64)
+    saved = ctx.tuning()
+    try:
+        for identity in (50, 90):
+            ctx.set_tuning([0] * 17)
+            db.graph(identity, 64, ng.GRAPH_BATCHED_EAGER)
+            keys0, sc0 = db.pair_scores()
+            edges0 = db.edges()
+            ctx.set_tuning([0, 0] + [1 << 30] * 15)  # every window bin -> k_align_wave<1>
+            st = db.graph(identity, 64, ng.GRAPH_BATCHED_EAGER)
+            keys, sc = db.pair_scores()
+            np.testing.assert_array_equal(keys, keys0)
+            np.testing.assert_array_equal(sc, sc0)
+            np.testing.assert_array_equal(db.edges(), edges0)
+            bins = db.bin_stats()
+            assert all(b["pairs"] == 0 for b in bins if b["kind"] == "window")
+            assert sum(b["ref_tiles"] for b in bins) == st["ref_tiles"]
+            jobs = np.zeros((len(keys), 3), dtype=np.uint32)
+            jobs[:, 0] = keys >> np.uint64(32)
+            jobs[:, 1] = keys & np.uint64(0xFFFFFFFF)
+            thr = float(np.float32(identity) / np.float32(100))
+            np.testing.assert_array_equal(sc, orc.align_jobs(d["packed"], d["word_off"], jobs, thr)[:, 2])
+    finally:
+        ctx.set_tuning(saved)
+
+
+def test_autotune_installs_crossovers(ctx):
+    say("autotune")
+    pts = ctx.autotune(90)
+    widths = sorted({p["width"] for p in pts})
+    assert widths == list(range(3, 17))  # W = 2 cannot pass the length test
+    assert all(p["window_ms"] > 0 and p["wave_ms"] > 0 for p in pts)
+    t = ctx.tuning()
+    for W in widths:
+        mine = [p for p in pts if p["width"] == W]
+        won = [p["pairs"] for p in mine if p["window_ms"] <= p["wave_ms"]]
+        assert t[W] == (min(won) if won else 4 * 4096)
+    ctx.set_tuning([0] * 17)
