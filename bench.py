@@ -273,8 +273,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=4, help="generator config of the workload")
-    ap.add_argument("--also-config", type=int, default=2,
-                    help="second workload reported beside the first (0: none)")
+    ap.add_argument("--also-configs", default="2:90,3:97",
+                    help="further workloads reported beside the first, config:identity,... "
+                         "('' for none)")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--identity", type=int, default=90)
@@ -393,26 +394,36 @@ def main():
                              "offered_load": mid["offered_load"],
                              "queries_per_s": mid["queries_per_s"]}
 
-    if args.also_config:
-        p2 = ng.gen_config(args.also_config, args.scale, args.seed)
+    for spec in [x for x in args.also_configs.split(",") if x]:
+        cfg, ident = (int(v) for v in spec.split(":"))
+        p2 = ng.gen_config(cfg, args.scale, args.seed)
         d2 = ng.gen_packed(p2)
-        m2 = measure(ng, np, ctx, info, d2, args.identity, args.steps, args.warmup,
-                     0 if args.no_plain else 3, "config %d" % args.also_config)
+        m2 = measure(ng, np, ctx, info, d2, ident, args.steps, args.warmup,
+                     0 if args.no_plain else (3 if cfg != 3 else 1), "config %d" % cfg)
         rows2, phase2 = align_rooflines(m2, peak)
-        o2 = {"workload": "%d proteins in %d families, log-normal lengths (median %g, max %d), "
-                          "-i %d" % (d2.n, p2.n_families, p2.len_a, p2.len_max, args.identity),
-              "value": round(m2["value"], 1), "ms_per_step": round(m2["ms_per_step"], 3),
+        kind = "DNA (ACGT)" if p2.dna else "proteins"
+        lens = ("normal lengths (mean %g, sd %g, %d-%d)" % (p2.len_a, p2.len_b, p2.len_min, p2.len_max)
+                if p2.len_normal else "log-normal lengths (median %g, max %d)" % (p2.len_a, p2.len_max))
+        o2 = {"workload": "%d %s in %d families, %s, -i %d" % (d2.n, kind, p2.n_families, lens, ident),
+              "identity": ident, "value": round(m2["value"], 1),
+              "ms_per_step": round(m2["ms_per_step"], 3),
+              "residues": int(d2.lengths.astype(np.int64).sum()),
               "stage_ms": {"sketch": round(m2["sketch_ms"], 3), "bucket": round(m2["bucket_ms"], 3),
                            "graph": round(m2["graph_ms"], 3),
                            "finalize": round(m2["finalize_ms"], 3)},
               "pairs": {"candidates_with_duplicates": m2["stats"]["total_pairs"],
                         "distinct": m2["stats"]["distinct_pairs"],
                         "edges": m2["stats"]["n_edges"], "ref_tiles": m2["stats"]["ref_tiles"]},
+              "aligned_pairs_per_s": round(m2["stats"]["total_pairs"] /
+                                           ((m2["graph_ms"] + m2["finalize_ms"]) * 1e-3), 1),
               "align_phase": phase2, "align_kernels": rows2[:4]}
+        ops2, nbytes2 = sketch_work(d2.lengths)
+        o2["sketch_hbm_gbps"] = round(nbytes2 / (m2["sketch_ms"] * 1e-3) / 1e9, 2)
         if "plain" in m2:
             o2["plain_form"] = m2["plain"]
             o2["batched_vs_plain"] = round(m2["value"] / m2["plain"]["value"], 3)
-        out["config_%d" % args.also_config] = o2
+        out["config_%d" % cfg] = o2
+        del d2
 
     ctx.close()
     if not args.no_cpu:

@@ -26,7 +26,12 @@ __global__ __launch_bounds__(256) void k_sketch(const uint8_t* __restrict__ code
   const uint32_t seq = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (seq >= n) return;
   const uint32_t L = __builtin_amdgcn_readfirstlane(lengths[seq]);
-  const uint32_t* cw = (const uint32_t*)(codes + __builtin_amdgcn_readfirstlane(code_off[seq]));
+  // readfirstlane moves 32 bits: the 64-bit offset goes as two halves (code
+  // areas pass 2^31 bytes at a few million sequences).
+  const uint64_t co = code_off[seq];
+  const uint32_t co_lo = __builtin_amdgcn_readfirstlane((uint32_t)co);
+  const uint32_t co_hi = __builtin_amdgcn_readfirstlane((uint32_t)(co >> 32));
+  const uint32_t* cw = (const uint32_t*)(codes + (((uint64_t)co_hi << 32) | co_lo));
   const uint32_t seed = lane * 0x9e3779b1u + 0x85ebca6bu;
   const uint32_t s1 = seed ^ 61u ^ (seed >> 16);
   uint32_t mn = 0xFFFFFFFFu;

@@ -341,3 +341,56 @@ def test_autotune_installs_crossovers(ctx):
         won = [p["pairs"] for p in mine if p["window_ms"] <= p["wave_ms"]]
         assert t[W] == (min(won) if won else 4 * 4096)
     ctx.set_tuning([0] * 17)
+
+
+@pytest.fixture(scope="module")
+def dnadb():
+    """The DNA families of the benchmark suite (ACGT, ~1400 nt, 0-5% substitutions),
+    scaled to 60 families of 100."""
+    p = ng.gen_config(3, scale=0.003, seed=4)
+    d = ng.gen_packed(p)
+    sig = orc.sketch_all(d.packed, d.word_off)
+    rep = orc.bucket(sig)
+    want = {i: orc.build_graph(d.packed, d.word_off, d.lengths, rep, 64, i)[0] for i in (90, 97)}
+    return d, sig, rep, want
+
+
+def test_dna_sketch_bucket(ctx, dnadb):
+    d, sig, rep, _ = dnadb
+    say("dna sketch/bucket", d.n)
+    db = ng.Database(ctx, d.packed, d.word_off, d.lengths)
+    db.sketch()
+    db.bucket()
+    np.testing.assert_array_equal(db.sigs(), sig)
+    np.testing.assert_array_equal(db.reps(), rep)
+
+
+@pytest.mark.parametrize("identity", [90, 97])
+@pytest.mark.parametrize("mode", [ng.GRAPH_PLAIN, ng.GRAPH_BATCHED])
+def test_dna_graph(ctx, dnadb, identity, mode):
+    d, sig, rep, want = dnadb
+    say("dna graph", identity, mode)
+    db = ng.Database(ctx, d.packed, d.word_off, d.lengths)
+    db.set_reps(rep, 64)
+    db.graph(identity, 64, mode)
+    np.testing.assert_array_equal(db.edges(), want[identity])
+    assert len(want[identity]) > 500
+
+
+def test_sketch_past_2gib_of_codes(ctx):
+    """Sequences whose byte codes start beyond 2^31 bytes of the code area
+    (1.6M DNA sequences of ~1400 nt): their signatures match the oracle."""
+    p = ng.gen_config(3, scale=0.8, seed=6)
+    d = ng.gen_packed(p)
+    codes_bytes = int((32 * ((d.lengths.astype(np.int64) + 31) // 32)).sum())
+    assert codes_bytes > (1 << 31) + (1 << 24)
+    say("sketch past 2 GiB", d.n, codes_bytes)
+    db = ng.Database(ctx, d.packed, d.word_off, d.lengths)
+    db.sketch()
+    got = db.sigs()
+    db.close()
+    tail = np.arange(d.n - 2000, d.n)
+    head = np.arange(0, 2000)
+    for idx in (head, tail):
+        want = np.stack([orc.sketch_record(d.packed[int(d.word_off[i]):]) for i in idx])
+        np.testing.assert_array_equal(got[idx], want)
