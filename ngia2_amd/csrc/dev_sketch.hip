@@ -55,11 +55,29 @@ __global__ __launch_bounds__(256) void k_sketch(const uint8_t* __restrict__ code
     const uint32_t w = __builtin_amdgcn_readfirstlane(cw[j >> 2]);
     kmer = (kmer << 4) + ((w >> (8 * (j & 3))) & 0xFu);
   }
-  // Head up to a 4-aligned position, then four positions per code dword.
-  for (; j < L && (j & 3); j++) {
+  // Head up to a 16-aligned position, then sixteen positions per four-dword
+  // scalar load: one wait on the scalar cache per sixteen windows.  The load
+  // reads whole 16-byte groups; a record's codes are padded to 32 bytes, so
+  // it stays within the record's own padding.
+  for (; j < L && (j & 15); j++) {
     const uint32_t w = __builtin_amdgcn_readfirstlane(cw[j >> 2]);
     kmer = (kmer << 4) + ((w >> (8 * (j & 3))) & 0xFu);
     hash(kmer ^ (kmer >> 16));
+  }
+  for (; j + 16 <= L; j += 16) {
+    const uint4 c4 = *(const uint4*)(cw + (j >> 2));
+    const uint32_t ws[4] = {(uint32_t)__builtin_amdgcn_readfirstlane(c4.x),
+                            (uint32_t)__builtin_amdgcn_readfirstlane(c4.y),
+                            (uint32_t)__builtin_amdgcn_readfirstlane(c4.z),
+                            (uint32_t)__builtin_amdgcn_readfirstlane(c4.w)};
+    #pragma unroll
+    for (int q = 0; q < 4; q++) {
+      #pragma unroll
+      for (int m = 0; m < 4; m++) {
+        kmer = (kmer << 4) + ((ws[q] >> (8 * m)) & 0xFu);
+        hash(kmer ^ (kmer >> 16));
+      }
+    }
   }
   for (; j + 4 <= L; j += 4) {
     const uint32_t w = __builtin_amdgcn_readfirstlane(cw[j >> 2]);
