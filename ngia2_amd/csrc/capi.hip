@@ -298,6 +298,8 @@ extern "C" int ngia_open(int device, ngia_ctx** out) {
     ctx->bucket_wide = bw && bw[0] == '1';
     const char* sb = std::getenv("NGIA_SERIAL_BINS");
     ctx->serial_bins = sb && sb[0] == '1';
+    const char* rl = std::getenv("NGIA_RAW_LCS");
+    ctx->raw_lcs = rl && rl[0] == '1';
     NGIA_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (auto& e : ctx->ev) NGIA_HIP(hipEventCreate(&e));
     for (auto& q : ctx->side) NGIA_HIP(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));

@@ -80,6 +80,7 @@ struct ngia_ctx {
   // NGIA_SERIAL_BINS=1 runs the alignment bins one after another on the
   // context stream (each bin's kernel time is then its time alone)
   bool serial_bins = false;
+  bool raw_lcs = false;  // env NGIA_RAW_LCS=1: alignment kernels store LCS lengths, not scores
   ngia_timing timing{};    // device time of the last sketch / bucket / graph
   // live handles made on this context; ngia_close detaches them, so a handle
   // destroyed after its context only frees its own memory
@@ -231,6 +232,7 @@ inline AlignArgs align_args(const ngia_db* db, float thr) {
   a.codes = db->codes.as<uint8_t>();
   a.code_off = db->code_off.as<uint64_t>();
   a.thr = thr;
+  a.raw_lcs = db->ctx && db->ctx->raw_lcs ? 1u : 0u;
   return a;
 }
 

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(64) void k_align_plain(AlignArgs A, uint32_t* __res
     }
     uint32_t lcs = 0;
     for (uint32_t w = 0; w < nw; w++) lcs += 32u - __popc(scratch[(uint64_t)w * stride + tid]);
-    jobs[3 * job + 2] = lcs_score(lcs, R, T, A.thr);
+    jobs[3 * job + 2] = pair_value(A, lcs, R, T);
   }
 }
 
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(64) void k_align_win(AlignArgs A, const uint2* __re
     for (uint32_t cb0 = 0; cb0 < nb; cb0 += W) win_round<W, 0>(st, cb0);
     #pragma unroll
     for (int j = 0; j < W; j++) st.lcs += 32u - __popc(rows[j]);
-    score[p] = lcs_score(st.lcs, R, T, A.thr);
+    score[p] = pair_value(A, st.lcs, R, T);
   }
 }
 
@@ -473,7 +473,7 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
     for (int m = 0; m < K; m++) part += 32u - __popc(rows[m]);
     for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
     lcs += part;
-    if (lane == 0) score[p] = lcs_score(lcs, R, T, A.thr);
+    if (lane == 0) score[p] = pair_value(A, lcs, R, T);
     lds_order();
   }
 }

@@ -84,7 +84,15 @@ struct AlignArgs {
   const uint8_t* codes;
   const uint64_t* code_off;
   float thr;
+  uint32_t raw_lcs = 0;  // nonzero: store the LCS length instead of the score
 };
+// What an alignment kernel stores for a pair it aligned: the score, or the
+// LCS length itself when raw_lcs is set (a check of the recurrence that does
+// not go through the score's integer rounding and threshold).
+__host__ __device__ inline uint32_t pair_value(const AlignArgs& A, uint32_t lcs, uint32_t R,
+                                               uint32_t T) {
+  return A.raw_lcs ? lcs : lcs_score(lcs, R, T, A.thr);
+}
 // Plain per-band form: one thread per job triplet, any length.  The rep of
 // every job has at most nw_max words (ceil(R/32)).  With count_ptr set the
 // job count is read from device memory (count is then an upper bound).

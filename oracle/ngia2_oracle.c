@@ -167,8 +167,7 @@ static void band_edges(uint32_t R, uint32_t T, float thr, int fused,
 }
 
 /* B6: kernel_dynamic, buildGraph.hpp:195-252, as the same 32x32 tile loop. */
-uint32_t orc_align_record(const uint32_t* represent, const uint32_t* target, float thr,
-                          int fused) {
+uint32_t orc_align_lcs(const uint32_t* represent, const uint32_t* target, float thr, int fused) {
   const uint32_t R = represent[0], T = target[0];
   if ((float)T < (float)R * thr) return 0;  /* job keeps its initial 0 */
   const uint32_t nw = (R + 31) / 32;
@@ -211,6 +210,14 @@ uint32_t orc_align_record(const uint32_t* represent, const uint32_t* target, flo
   }
   uint32_t lcs = 0;
   for (uint32_t w = 0; w < nw; w++) lcs += 32u - (uint32_t)__builtin_popcount(lines[w]);
+  return lcs;
+}
+
+uint32_t orc_align_record(const uint32_t* represent, const uint32_t* target, float thr,
+                          int fused) {
+  const uint32_t R = represent[0], T = target[0];
+  if ((float)T < (float)R * thr) return 0;  /* job keeps its initial 0 */
+  const uint32_t lcs = orc_align_lcs(represent, target, thr, fused);
   uint32_t denom = R + T - lcs;
   if ((float)lcs > (float)denom * thr) return lcs * 100u / denom;
   return 0;
@@ -238,6 +245,16 @@ void orc_align_jobs(const uint32_t* packed, const uint64_t* word_off, uint32_t* 
   for (int64_t j = 0; j < (int64_t)count; j++) {
     uint32_t* job = jobs + 3 * (size_t)j;
     job[2] = orc_align_record(packed + word_off[job[0]], packed + word_off[job[1]], thr, fused);
+  }
+}
+
+void orc_align_jobs_lcs(const uint32_t* packed, const uint64_t* word_off, uint32_t* jobs,
+                        uint32_t count, float thr, int fused, int nthreads) {
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+  #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+  for (int64_t j = 0; j < (int64_t)count; j++) {
+    uint32_t* job = jobs + 3 * (size_t)j;
+    job[2] = orc_align_lcs(packed + word_off[job[0]], packed + word_off[job[1]], thr, fused);
   }
 }
 

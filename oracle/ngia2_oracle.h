@@ -26,6 +26,9 @@
  *                              passing pairs; identical / empty / filtered
  *                              special cases; fused vs unfused band edges give
  *                              identical scores on the fixtures
+ *   orc_align_lcs              the LCS length orc_align_record scores; a
+ *                              per-bit restatement of the banded recurrence
+ *                              in the tests (band-binding pairs included)
  *   orc_build_graph            sort/unique brute force over per-band scores
  *   orc_leiden                 reference LeidenOmp build (oracle/_ref) on the
  *                              same edge lists; mt19937_64 known-answer value
@@ -68,11 +71,16 @@ uint64_t orc_candidates(const uint32_t* rep_row, const uint32_t* len, uint32_t n
                         uint32_t identity, uint64_t* out);
 /* buildGraph.hpp:195-252 on two packed records; returns the job score */
 uint32_t orc_align_record(const uint32_t* rep, const uint32_t* tar, float thr, int fused);
+/* The LCS length of the banded recurrence (0 when the length filter rejects). */
+uint32_t orc_align_lcs(const uint32_t* rep, const uint32_t* tar, float thr, int fused);
 /* number of 32x32 tiles the reference loop visits for (R,T,thr) */
 uint64_t orc_align_tiles(uint32_t R, uint32_t T, float thr, int fused);
 /* kernel_dynamic contract: jobs[3i+2] = score of (jobs[3i], jobs[3i+1]) */
 void orc_align_jobs(const uint32_t* packed, const uint64_t* word_off, uint32_t* jobs,
                     uint32_t count, float thr, int fused, int nthreads);
+/* the same with jobs[3i+2] = orc_align_lcs of the pair */
+void orc_align_jobs_lcs(const uint32_t* packed, const uint64_t* word_off, uint32_t* jobs,
+                        uint32_t count, float thr, int fused, int nthreads);
 /* textbook LCS on residue codes; code 0 never matches (buildGraph.hpp:220-225) */
 uint32_t orc_lcs_full(const uint8_t* a, uint32_t R, const uint8_t* b, uint32_t T);
 /* buildGraph.hpp:256-363: full band loop, sorted unique edges (malloc'd) */

@@ -43,6 +43,9 @@ def orc():
         L.orc_align_tiles.restype = U64
         L.orc_align_tiles.argtypes = [U32, U32, C.c_float, C.c_int]
         L.orc_align_jobs.argtypes = [P, P, P, U32, C.c_float, C.c_int, C.c_int]
+        L.orc_align_lcs.restype = U32
+        L.orc_align_lcs.argtypes = [P, P, C.c_float, C.c_int]
+        L.orc_align_jobs_lcs.argtypes = [P, P, P, U32, C.c_float, C.c_int, C.c_int]
         L.orc_lcs_full.restype = U32
         L.orc_lcs_full.argtypes = [P, U32, P, U32]
         L.orc_build_graph.restype = C.c_int
@@ -136,16 +139,23 @@ def align_record(rep_rec, tar_rec, thr, fused=1):
     return orc().orc_align_record(_p(a), _p(b), C.c_float(thr), fused)
 
 
+def align_lcs(rep_rec, tar_rec, thr, fused=1):
+    a = np.ascontiguousarray(rep_rec, dtype=np.uint32)
+    b = np.ascontiguousarray(tar_rec, dtype=np.uint32)
+    return orc().orc_align_lcs(_p(a), _p(b), C.c_float(thr), fused)
+
+
 def align_tiles(R, T, thr, fused=1):
     return orc().orc_align_tiles(R, T, C.c_float(thr), fused)
 
 
-def align_jobs(packed, word_off, jobs, thr, fused=1, threads=0):
+def align_jobs(packed, word_off, jobs, thr, fused=1, threads=0, raw_lcs=False):
+    """Score (or, with raw_lcs, the banded LCS length) of every job pair."""
     packed = np.ascontiguousarray(packed, dtype=np.uint32)
     word_off = np.ascontiguousarray(word_off, dtype=np.uint64)
     jobs = np.array(jobs, dtype=np.uint32, order="C", copy=True)
-    orc().orc_align_jobs(_p(packed), _p(word_off), _p(jobs), len(jobs), C.c_float(thr), fused,
-                         threads)
+    fn = orc().orc_align_jobs_lcs if raw_lcs else orc().orc_align_jobs
+    fn(_p(packed), _p(word_off), _p(jobs), len(jobs), C.c_float(thr), fused, threads)
     return jobs
 
 

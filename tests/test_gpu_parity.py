@@ -9,7 +9,7 @@ import pytest
 
 import ngia2_amd as ng
 import oracle_lib as orc
-from conftest import AMINO, family_fasta
+from conftest import AMINO, family_fasta, mutate
 
 pytestmark = pytest.mark.gpu
 
@@ -195,6 +195,93 @@ def test_all_pairs_long(ctx, longdb):
         thr = float(np.float32(identity) / np.float32(100))
         want = orc.align_jobs(d["packed"], d["word_off"], jobs, thr)
         np.testing.assert_array_equal(sc, want[:, 2])
+
+
+def band_edge_groups(rng):
+    """Groups of sequences whose pairs make the band edges matter.
+
+    Per base length: the ancestor, same-length copies shifted left and right
+    by up to a quarter of the length (the alignment runs along or outside a
+    band edge), copies with a junk prefix, substituted copies, unrelated
+    sequences, and low-complexity sequences (one letter with a few others at
+    random places), whose band-edge words decide single matches.
+    """
+    groups = []
+    for L in (90, 400, 1100, 2300, 5000, 9000):
+        anc = bytes(AMINO[i] for i in rng.integers(20, size=L))
+        g = [anc]
+        for _ in range(3):
+            sh = int(rng.integers(1, L // 4 + 2))
+            junk = bytes(AMINO[i] for i in rng.integers(20, size=sh))
+            g.append(anc[sh:] + junk)       # path offset +sh, same length
+            g.append(junk + anc[:L - sh])   # path offset -sh, same length
+            g.append(anc[sh:])              # shorter suffix
+        g.append(mutate(rng, anc, 0.05, 0.01))
+        g.append(mutate(rng, anc, 0.3, 0.02))
+        for _ in range(2):
+            g.append(bytes(AMINO[i] for i in rng.integers(20, size=int(L * rng.uniform(0.8, 1.0)))))
+        for _ in range(4):
+            n = int(L * rng.uniform(0.85, 1.0))
+            s = bytearray(b"A" * n)
+            for p in rng.integers(0, n, size=max(1, n // 200)):
+                s[p] = AMINO[int(rng.integers(1, 20))]
+            g.append(bytes(s))
+        groups.append(g)
+    # widths that need K = 32 at low identity
+    anc = bytes(AMINO[i] for i in rng.integers(20, size=20000))
+    groups.append([anc, anc[3000:] + anc[:3000], mutate(rng, anc, 0.4, 0.02)])
+    return groups
+
+
+def test_band_edge_pairs(ctx):
+    """Every ordered pair within each group through the batched bins (K = 1..32)."""
+    rng = np.random.default_rng(29)
+    groups = band_edge_groups(rng)
+    seqs, gid = [], []
+    for k, g in enumerate(groups):
+        g = sorted(g, key=lambda s: -len(s))  # the earlier record is the longer one
+        seqs += g
+        gid += [k] * len(g)
+    recs = [orc.pack(s) for s in seqs]
+    off = np.cumsum([0] + [len(r) for r in recs[:-1]]).astype(np.uint64)
+    packed = np.concatenate(recs).astype(np.uint32)
+    lengths = np.array([len(s) for s in seqs], dtype=np.uint32)
+    n = len(seqs)
+    gid = np.array(gid)
+    rows = []
+    for b in range(64):
+        r = np.arange(n, dtype=np.int64) - 1 - b
+        ok = (r >= 0) & (gid[np.maximum(r, 0)] == gid)
+        r[~ok] = np.arange(n)[~ok]
+        rows.append(r.astype(np.uint32))
+    rep = np.stack(rows)
+    # A second context stores LCS lengths instead of scores: the band edges
+    # decide the LCS of pairs that fail the identity test, whose score is 0
+    # whatever the LCS, so only the raw value checks the edges exactly.
+    os.environ["NGIA_RAW_LCS"] = "1"
+    try:
+        raw_ctx = ng.Context(0)
+    finally:
+        del os.environ["NGIA_RAW_LCS"]
+    kinds = set()
+    with raw_ctx:
+        for identity in (5, 30, 60, 90, 99):
+            thr = float(np.float32(identity) / np.float32(100))
+            for c, raw in ((ctx, False), (raw_ctx, True)):
+                say("band edge pairs, identity", identity, "records", n, "raw lcs", raw)
+                db = ng.Database(c, packed, off, lengths)
+                db.set_reps(rep, 64)
+                db.graph(identity, 64, ng.GRAPH_BATCHED)
+                keys, sc = db.pair_scores()
+                kinds |= {(b["kind"], b["width"]) for b in db.bin_stats() if b["pairs"]}
+                jobs = np.zeros((len(keys), 3), dtype=np.uint32)
+                jobs[:, 0] = keys >> np.uint64(32)
+                jobs[:, 1] = keys & np.uint64(0xFFFFFFFF)
+                want = orc.align_jobs(packed, off, jobs, thr, raw_lcs=raw)
+                np.testing.assert_array_equal(sc, want[:, 2])
+                db.close()
+    waves = {w for k, w in kinds if k == "wave"}
+    assert {1, 2, 4, 8, 16, 32} <= waves, kinds
 
 
 def test_makedb_bytes(ctx, c1db, tmp_path):

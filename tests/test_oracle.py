@@ -169,6 +169,76 @@ def test_align_tile_loop_equals_textbook_lcs_score():
     assert checked > 200  # the property was exercised on passing pairs
 
 
+def banded_lcs_bigint(a, b, thr32):
+    """The reference's banded recurrence (buildGraph.hpp:195-246) on whole-band
+    integers: per target character the band's rep bits form one Python
+    integer, the addition carries across word boundaries by itself, and the
+    carry out of the band's top word is masked off.  Band edges as the fused
+    float32 expressions: T - T*thr and R - T*thr rounded once."""
+    R, T = len(a), len(b)
+    if np.float32(T) < np.float32(R) * thr32:
+        return 0
+    l32 = np.float32(T - T * np.float64(thr32))
+    r32 = np.float32(R - T * np.float64(thr32))
+    lsft = (max(int(np.ceil(l32)), 0) + 31) & ~31
+    rsft = max(int(np.ceil(r32)), 0) + 32
+    nw = (R + 31) // 32
+    masks = {}
+    for j, ch in enumerate(a):
+        c = py_code(ch)
+        if c:
+            masks[c] = masks.get(c, 0) | (1 << j)
+    V = (1 << (32 * nw)) - 1
+    for i in range(T):
+        blk = i // 32
+        js = max(32 * blk - lsft, 0)
+        je = min(32 * blk + rsft, R)
+        if je <= js:
+            continue
+        width = 32 * ((je + 31) // 32) - js
+        mask = (1 << width) - 1
+        v = (V >> js) & mask
+        m = (masks.get(py_code(b[i]), 0) >> js) & mask if py_code(b[i]) else 0
+        nv = ((v + (v & m)) | (v & ~m)) & mask
+        V = (V & ~(mask << js)) | (nv << js)
+    return 32 * nw - bin(V).count("1")
+
+
+def test_align_lcs_whole_band_restatement():
+    """orc_align_lcs on pairs whose alignment runs along or outside the band
+    edges (shifted, unrelated, low-complexity), where it differs from the
+    textbook LCS."""
+    rng = np.random.default_rng(31)
+    differs = 0
+    for trial in range(500):
+        L = int(rng.integers(1, 300))
+        anc = bytes(AMINO[i] for i in rng.integers(20, size=L))
+        kind = trial % 5
+        if kind == 0:
+            b = bytes(AMINO[i] for i in rng.integers(20, size=int(rng.integers(1, L + 1))))
+        elif kind == 1:
+            sh = int(rng.integers(0, L // 2 + 1))
+            b = anc[sh:] + bytes(AMINO[i] for i in rng.integers(20, size=sh))
+        elif kind == 2:
+            sh = int(rng.integers(0, L // 2 + 1))
+            b = bytes(AMINO[i] for i in rng.integers(20, size=sh)) + anc[:L - sh]
+        elif kind == 3:
+            anc = b"".join(b"A" if rng.random() < 0.9 else AMINO[i:i + 1] for i in rng.integers(20, size=L))
+            b = mutate(rng, anc, 0.1, 0.05)
+        else:
+            b = mutate(rng, anc, float(rng.uniform(0, 0.5)), float(rng.uniform(0, 0.1)))
+        a = anc
+        if len(b) > len(a):
+            a, b = b, a
+        thr = np.float32(int(rng.integers(1, 100))) / np.float32(100)
+        want = banded_lcs_bigint(a, b, thr)
+        got = orc.align_lcs(orc.pack(a), orc.pack(b), float(thr))
+        assert got == want, (trial, len(a), len(b), float(thr))
+        if len(a) and np.float32(len(b)) >= np.float32(len(a)) * thr:
+            differs += got != orc.lcs_full(codes_of(a), codes_of(b))
+    assert differs > 10  # the band bound the recurrence on many pairs
+
+
 def test_fused_and_unfused_band_edges_score_alike():
     rng = np.random.default_rng(13)
     for trial in range(800):
