@@ -414,12 +414,25 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
     for (uint32_t cb = 0; cb < nb; cb++) {
       const uint32_t nxt = (lane < 8 && cb + 1 < nb) ? tcw[8 * (cb + 1) + lane] : 0u;
       const uint32_t* pqb = peq + vs;
-      for (int g = 0; g < 8; g++) {
+      // The masks of group g's four characters; for K <= 8 those of group
+      // g + 1 are loaded while group g is computed (two buffers, g mod 2), so
+      // that only the block's first group waits for its loads.
+      constexpr bool kAhead = K <= 8;
+      uint32_t Mb[kAhead ? 2 : 1][4][K];
+      auto load_group = [&](int g, uint32_t (&Mx)[4][K]) {
         const uint32_t cw = __builtin_amdgcn_readlane(cur, g);
-        // the masks of the four characters are loaded before any is used
-        uint32_t Mq[4][K];
         #pragma unroll
-        for (int mm = 0; mm < 4; mm++) load_masks<K>(pqb + ((cw >> (8 * mm)) & 0xFu) * S, Mq[mm]);
+        for (int mm = 0; mm < 4; mm++) load_masks<K>(pqb + ((cw >> (8 * mm)) & 0xFu) * S, Mx[mm]);
+      };
+      if constexpr (kAhead) load_group(0, Mb[0]);
+      #pragma unroll
+      for (int g = 0; g < 8; g++) {
+        uint32_t(&Mq)[4][K] = Mb[kAhead ? (g & 1) : 0];
+        if constexpr (kAhead) {
+          if (g < 7) load_group(g + 1, Mb[(g + 1) & 1]);
+        } else {
+          load_group(g, Mq);
+        }
         #pragma unroll
         for (int mm = 0; mm < 4; mm++) {
           uint32_t(&M)[K] = Mq[mm];
