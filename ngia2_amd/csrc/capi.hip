@@ -142,32 +142,38 @@ void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ban
             ctx->wave_kmax, dc->remap, ctx->route, grid, s);
   const AlignArgs aa = align_args(db, thr);
   // The bins run concurrently, forked onto the side streams and joined back.
-  // The pairs of the wide wave bins are the longest, so the bins start
-  // widest first: K >= 8 on side stream 0, K = 4 alone on side stream 1
-  // (in heavy-tailed sets it holds the longest pairs), K = 2 and 1 on side
-  // stream 2, the window bins spread over streams 3, 0 and 2.
+  // Every wave bin below K = 16 is latency bound (one pair per wavefront, the
+  // bin lasts as long as its longest pair), so each gets a side stream of its
+  // own and starts at the fork: K = 8 on side stream 0, K = 4 on 1, K = 2 on
+  // 2, K = 1 on 3.  The window bins (throughput bound) follow round-robin on
+  // side streams 0, 3 and 2, widest first.  K = 32 and 16 (128 and 64 KiB of
+  // LDS per block) run before the fork: launched beside the others, their
+  // blocks wait for a CU with that much free LDS, which the other bins'
+  // persistent blocks hold until they finish, and that wait would stall
+  // their stream even when the bin is empty.
   if (time_align) NGIA_HIP(hipEventRecord(ctx->ev[2], s));
   const bool fork = !ctx->serial_bins;
-  if (fork) {
-    NGIA_HIP(hipEventRecord(ctx->fork, s));
-    for (int k = 0; k < kSideStreams; k++) NGIA_HIP(hipStreamWaitEvent(ctx->side[k], ctx->fork, 0));
-  }
   auto mark = [&](int bin, int end, hipStream_t q) {
     if (time_align) NGIA_HIP(hipEventRecord(ctx->bin_ev[2 * bin + end], q));
   };
-  auto wave_stream = [&](int k) { return fork ? ctx->side[k >= 3 ? 0 : k == 2 ? 1 : 2] : s; };
-  static constexpr int kWinStreams[3] = {3, 0, 2};
-  int slot = 0;
-  for (int k = 5; k >= 0; k--) {
-    if ((1u << k) > ctx->wave_kmax) continue;
+  auto wave = [&](int k, hipStream_t q) {
+    if ((1u << k) > ctx->wave_kmax) return;
     const int bin = kBinWave0 + k;
-    hipStream_t q = wave_stream(k);
     mark(bin, 0, q);
     launch_align_wave(1 << k, aa, db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[bin],
                       &dc->bin_cnt[bin], &dc->work[bin], db->score.as<uint32_t>(),
                       persistent_blocks(ctx, wave_lds_bytes(1 << k)), q);
     mark(bin, 1, q);
+  };
+  wave(5, s);
+  wave(4, s);
+  if (fork) {
+    NGIA_HIP(hipEventRecord(ctx->fork, s));
+    for (int k = 0; k < kSideStreams; k++) NGIA_HIP(hipStreamWaitEvent(ctx->side[k], ctx->fork, 0));
   }
+  for (int k = 3; k >= 0; k--) wave(k, fork ? ctx->side[3 - k] : s);
+  static constexpr int kWinStreams[3] = {0, 3, 2};
+  int slot = 0;
   for (int W = kWinMax; W >= 2; W--) {
     hipStream_t q = fork ? ctx->side[kWinStreams[slot++ % 3]] : s;
     mark(W, 0, q);
