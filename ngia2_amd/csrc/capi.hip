@@ -202,26 +202,44 @@ void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ban
                  db->escores.as<uint32_t>(), &dc->ecount, grid, s);
 }
 
-void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
-                   ngia_graph_stats* st, bool eager) {
-  hipStream_t s = ctx->stream;
+// Workspace of one batched round over db (sized for every target).
+void ensure_batched_space(ngia_db* db, uint32_t bands) {
   const uint32_t n = db->n;
-  const uint64_t cap = (uint64_t)n * bands;  // distinct pairs <= candidates <= n*bands
+  const uint64_t cap = std::max<uint64_t>((uint64_t)n * bands, 1);  // distinct <= candidates <= n*bands
   db->counters.ensure(sizeof(Counters));
   db->cnt.ensure(4ull * n);
   db->off.ensure(8ull * (n + 1));
   db->sums.ensure(8 * scan_scratch_words(n));
-  db->pairs.ensure(8 * std::max<uint64_t>(cap, 1));
-  db->score.ensure(4 * std::max<uint64_t>(cap, 1));
-  db->bin_of.ensure(4 * std::max<uint64_t>(cap, 1));
-  db->list.ensure(4 * std::max<uint64_t>(cap, 1));
+  db->pairs.ensure(8 * cap);
+  db->score.ensure(4 * cap);
+  db->bin_of.ensure(4 * cap);
+  db->list.ensure(4 * cap);
   const uint32_t nw_max = (db->max_len + 31) / 32;
   // The plain bin only holds windows wider than 2048 words; its job list and
   // row scratch are sized for the whole pair list so the graph never grows.
-  db->plain_jobs.ensure(12 * std::max<uint64_t>(cap, 1));
+  db->plain_jobs.ensure(12 * cap);
   db->plain_scratch.ensure(4 * align_plain_scratch_words(nw_max));
   ensure_edge_space(db, cap);
-  const bool reuse = db->gexec && db->g_identity == identity && db->g_bands == bands;
+}
+
+// The batched round, captured once per (identity, bands) and replayed (or
+// launched eagerly with the alignment phase timed).
+void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
+                   ngia_graph_stats* st, bool eager) {
+  hipStream_t s = ctx->stream;
+  const uint32_t n = db->n;
+  ensure_batched_space(db, bands);
+  // Everything the captured kernels were given: a replay is valid while none
+  // of it changed (a grown buffer moves, a new tuning changes the routing).
+  std::vector<uintptr_t> key;
+  for (const DevBuf* b : {&db->packed, &db->word_off, &db->lengths, &db->codes, &db->code_off,
+                          &db->rep, &db->counters, &db->cnt, &db->off, &db->sums, &db->pairs,
+                          &db->score, &db->bin_of, &db->list, &db->plain_jobs, &db->plain_scratch,
+                          &db->ekeys, &db->escores})
+    key.push_back((uintptr_t)b->p);
+  for (uint32_t w : ctx->route.route_below) key.push_back(w);
+  const bool reuse = db->gexec && db->g_identity == identity && db->g_bands == bands &&
+                     db->g_key == key;
   NGIA_HIP(hipEventRecord(ctx->ev[0], s));
   if (eager) {
     // Same kernels without the graph, with events around the alignment phase.
@@ -246,6 +264,7 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
     NGIA_HIP(e);
     db->g_identity = identity;
     db->g_bands = bands;
+    db->g_key = key;
   }
   if (!eager) NGIA_HIP(hipGraphLaunch(db->gexec, s));
   NGIA_HIP(hipEventRecord(ctx->ev[1], s));
@@ -421,10 +440,6 @@ extern "C" int ngia_db_bucket(ngia_ctx* ctx, ngia_db* db) {
     NGIA_HIP(hipStreamSynchronize(ctx->stream));
     ctx->timing.bucket_ms = event_ms(ctx->ev[6], ctx->ev[7]);
     db->rep_bands = NGIA_BANDS;
-    if (db->gexec) {
-      (void)hipGraphExecDestroy(db->gexec);
-      db->gexec = nullptr;
-    }
     return NGIA_OK;
   });
 }

@@ -112,9 +112,11 @@ struct ngia_db {
   ngia::DevBuf flags, pos, sums, jobs, ekeys, escores, ekeys_tmp, escores_tmp, fin_temp;
   ngia::DevBuf counters, cnt, off, pairs, score, bin_of, list, plain_scratch, plain_jobs, bucket_scratch;
   uint64_t n_edges = 0, n_pairs = 0;
-  // the captured batched round
+  // the captured batched round; g_key holds the buffer addresses and the
+  // routing it was captured with
   hipGraphExec_t gexec = nullptr;
   uint32_t g_identity = 0, g_bands = 0;
+  std::vector<uintptr_t> g_key;
   // per-bin figures of the last batched graph
   bool have_bins = false;
   ngia::Counters last{};

@@ -49,23 +49,25 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
   if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
 }
 
-__global__ __launch_bounds__(1024) void k_scan_sums(uint64_t* __restrict__ sums, uint32_t nb) {
-  // single block: exclusive scan of nb block sums in chunks of 1024
-  __shared__ uint64_t sh[1024];
+// Single block of four wavefronts (a block that needs a whole CU's wave
+// slots waits behind any concurrent kernel that fills the CUs): exclusive
+// scan of nb block sums in chunks of 256.
+__global__ __launch_bounds__(256) void k_scan_sums(uint64_t* __restrict__ sums, uint32_t nb) {
+  __shared__ uint64_t sh[256];
   uint64_t carry = 0;
-  for (uint32_t c0 = 0; c0 < nb; c0 += 1024) {
+  for (uint32_t c0 = 0; c0 < nb; c0 += 256) {
     const uint32_t i = c0 + threadIdx.x;
     const uint64_t v = i < nb ? sums[i] : 0;
     sh[threadIdx.x] = v;
     __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
+    for (uint32_t off = 1; off < 256; off <<= 1) {
       const uint64_t t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
       __syncthreads();
       sh[threadIdx.x] += t;
       __syncthreads();
     }
     if (i < nb) sums[i] = carry + sh[threadIdx.x] - v;
-    const uint64_t tot = sh[1023];
+    const uint64_t tot = sh[255];
     __syncthreads();
     carry += tot;
   }
@@ -108,13 +110,13 @@ uint64_t scan_scratch_words(uint32_t n) { return (n + kScanTile - 1) / kScanTile
 void scan_u32(const uint32_t* in, uint32_t n, uint64_t* out, uint64_t* sums, hipStream_t s) {
   const uint32_t nb = (n + kScanTile - 1) / kScanTile;
   if (nb == 0) {
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, 0u);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, sums, 0u);
     // out[0] = 0 via a one-element copy of sums[0] (= 0 after k_scan_sums)
     NGIA_HIP(hipMemcpyAsync(out, sums, 8, hipMemcpyDeviceToDevice, s));
     return;
   }
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in, n, sums);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nb);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, s, sums, nb);
   hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, s, in, n, sums, out, nb);
   NGIA_HIP(hipGetLastError());
 }
@@ -443,20 +445,57 @@ void plain_bin_scores(const uint32_t* list, const uint32_t* bin_off, const uint3
   NGIA_HIP(hipGetLastError());
 }
 
-// Passing pairs -> raw edge list (order of the pair list, which is by target).
-__global__ void k_pairs_to_edges(const uint2* __restrict__ pairs, const uint64_t* __restrict__ npairs,
-                                 const uint32_t* __restrict__ score, uint64_t* __restrict__ ekeys,
-                                 uint32_t* __restrict__ escores,
-                                 unsigned long long* __restrict__ ecount) {
+// Passing pairs -> raw edge list.  A block takes tiles of 4096 pairs (16 per
+// thread), ranks its passing pairs with a block prefix sum and reserves their
+// slots with one global atomic per tile: passing pairs are a few percent, and
+// one atomic per wavefront on the shared counter serialised the kernel.  The
+// list's order is free (the sort + unique follows).
+__global__ __launch_bounds__(256) void k_pairs_to_edges(const uint2* __restrict__ pairs,
+                                                        const uint64_t* __restrict__ npairs,
+                                                        const uint32_t* __restrict__ score,
+                                                        uint64_t* __restrict__ ekeys,
+                                                        uint32_t* __restrict__ escores,
+                                                        unsigned long long* __restrict__ ecount) {
+  __shared__ uint32_t wsum[4];
+  __shared__ unsigned long long bbase;
   const uint64_t P = *npairs;
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
-       p += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t sc = score[p];
-    const unsigned long long at = wave_append(ecount, sc != 0);
-    if (sc == 0) continue;
-    const uint2 pr = pairs[p];
-    ekeys[at] = ((uint64_t)pr.x << 32) | pr.y;
-    escores[at] = sc;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (uint64_t base = (uint64_t)blockIdx.x * 4096; base < P; base += (uint64_t)gridDim.x * 4096) {
+    uint32_t sc[16], mine = 0;
+    #pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const uint64_t p = base + (uint64_t)k * 256 + tid;
+      sc[k] = p < P ? score[p] : 0u;
+      mine += sc[k] != 0u;
+    }
+    uint32_t x = mine;  // inclusive prefix over the wavefront
+    #pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t t = 0;
+      for (int w = 0; w < 4; w++) {
+        const uint32_t v = wsum[w];
+        wsum[w] = t;
+        t += v;
+      }
+      bbase = t ? atomicAdd(ecount, (unsigned long long)t) : 0ull;
+    }
+    __syncthreads();
+    unsigned long long at = bbase + wsum[wave] + x - mine;
+    #pragma unroll
+    for (int k = 0; k < 16; k++) {
+      if (sc[k] == 0u) continue;
+      const uint2 pr = pairs[base + (uint64_t)k * 256 + tid];
+      ekeys[at] = ((uint64_t)pr.x << 32) | pr.y;
+      escores[at] = sc[k];
+      at++;
+    }
+    __syncthreads();  // wsum and bbase are rewritten by the next tile
   }
 }
 

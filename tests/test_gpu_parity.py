@@ -416,6 +416,48 @@ def test_routed_window_bins_score_alike(ctx, c1db):
         ctx.set_tuning(saved)
 
 
+def test_captured_round_follows_tuning_and_rebucket(ctx, c1db):
+    """The captured batched round is replayed while everything it was captured
+    with stands (a fresh bucketing into the same rows included) and captured
+    again when the routing changes: the bins follow the new routing, and the
+    edges stay the oracle's throughout, with another database's rounds in
+    between."""
+    _, _, d = c1db
+    say("captured round: tuning change, rebucket, two databases")
+    want, _ = orc.build_graph(d["packed"], d["word_off"], d["lengths"], d["rep"], 64, 90)
+    db = ng.Database(ctx, d["packed"], d["word_off"], d["lengths"])
+    other = ng.Database(ctx, d["packed"], d["word_off"], d["lengths"])
+    other.set_reps(d["rep"], 64)
+    saved = ctx.tuning()
+    try:
+        ctx.set_tuning([0] * 17)
+        db.sketch()
+        db.bucket()
+        db.graph(90, 64, ng.GRAPH_BATCHED)
+        np.testing.assert_array_equal(db.edges(), want)
+        win = sum(b["pairs"] for b in db.bin_stats() if b["kind"] == "window")
+        assert win > 0
+        ctx.set_tuning([0, 0] + [1 << 30] * 15)  # every window bin -> k_align_wave<1>
+        db.graph(90, 64, ng.GRAPH_BATCHED)
+        np.testing.assert_array_equal(db.edges(), want)
+        assert all(b["pairs"] == 0 for b in db.bin_stats() if b["kind"] == "window")
+        other.graph(90, 64, ng.GRAPH_BATCHED)
+        np.testing.assert_array_equal(other.edges(), want)
+        ctx.set_tuning([0] * 17)
+        for _ in range(3):
+            db.sketch()
+            db.bucket()
+            db.graph(90, 64, ng.GRAPH_BATCHED)
+            np.testing.assert_array_equal(db.edges(), want)
+            assert sum(b["pairs"] for b in db.bin_stats() if b["kind"] == "window") == win
+            other.graph(90, 64, ng.GRAPH_BATCHED)
+            np.testing.assert_array_equal(other.edges(), want)
+    finally:
+        ctx.set_tuning(saved)
+        other.close()
+        db.close()
+
+
 def test_autotune_installs_crossovers(ctx):
     say("autotune")
     pts = ctx.autotune(90)
