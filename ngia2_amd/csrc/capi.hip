@@ -179,7 +179,7 @@ void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ban
     mark(W, 0, q);
     launch_align_win(W, aa, db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[W],
                      &dc->bin_cnt[W], &dc->work[W], db->score.as<uint32_t>(),
-                     persistent_blocks(ctx, win_lds_bytes(W)), q);
+                     persistent_blocks(ctx, win_lds_bytes(W, aa.slots != nullptr)), q);
     mark(W, 1, q);
   }
   for (int k = 0; fork && k < kSideStreams; k++) {
@@ -325,6 +325,8 @@ extern "C" int ngia_open(int device, ngia_ctx** out) {
     ctx->serial_bins = sb && sb[0] == '1';
     const char* rl = std::getenv("NGIA_RAW_LCS");
     ctx->raw_lcs = rl && rl[0] == '1';
+    const char* fa = std::getenv("NGIA_FULL_ALPHABET");
+    ctx->full_alphabet = fa && fa[0] == '1';
     NGIA_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (auto& e : ctx->ev) NGIA_HIP(hipEventCreate(&e));
     for (auto& q : ctx->side) NGIA_HIP(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));

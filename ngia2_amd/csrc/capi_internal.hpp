@@ -81,6 +81,9 @@ struct ngia_ctx {
   // context stream (each bin's kernel time is then its time alone)
   bool serial_bins = false;
   bool raw_lcs = false;  // env NGIA_RAW_LCS=1: alignment kernels store LCS lengths, not scores
+  // env NGIA_FULL_ALPHABET=1: the window kernels keep the 16-code table even
+  // for a compact alphabet
+  bool full_alphabet = false;
   ngia_timing timing{};    // device time of the last sketch / bucket / graph
   // live handles made on this context; ngia_close detaches them, so a handle
   // destroyed after its context only frees its own memory
@@ -108,6 +111,11 @@ struct ngia_db {
   uint32_t rep_bands = 0;
   bool have_sig = false;
   ngia::DevBuf packed, word_off, lengths, codes, code_off, sig, rep;
+  // compact alphabet (at most kSlotCodes nonzero codes occur): slot of every
+  // residue, laid out as codes; slot_code as in AlignArgs
+  bool compact = false;
+  ngia::DevBuf slot_codes;
+  uint32_t slot_code = 0;
   // graph workspace
   ngia::DevBuf flags, pos, sums, jobs, ekeys, escores, ekeys_tmp, escores_tmp, fin_temp;
   ngia::DevBuf counters, cnt, off, pairs, score, bin_of, list, plain_scratch, plain_jobs, bucket_scratch;
@@ -218,6 +226,36 @@ inline void db_upload(ngia_ctx* ctx, ngia_db* db, const uint32_t* packed, uint64
   h2d(db->code_off.p, coff.data(), 8ull * n, s);
   launch_codes(db->packed.as<uint32_t>(), db->word_off.as<uint64_t>(), db->code_off.as<uint64_t>(),
                n, db->codes.as<uint8_t>(), s);
+  // Which codes occur.  With at most kSlotCodes nonzero ones (DNA: A C G T,
+  // and N, which the code table reads as asparagine), the
+  // window kernels index their match-mask table by slot (code 0 -> slot 0, the
+  // present codes in ascending order -> 1..), a table of kSlotCodes + 1
+  // entries per word in place of 16.
+  db->compact = false;
+  db->slot_codes.release();
+  if (acc && !(ctx->full_alphabet)) {
+    DevBuf present;
+    present.ensure(4);
+    fill_u32(present.as<uint32_t>(), 0, 1, s);
+    launch_code_set(db->codes.as<uint8_t>(), acc, present.as<uint32_t>(), s);
+    uint32_t bits = 0;
+    d2h(&bits, present.p, 4, s);
+    NGIA_HIP(hipStreamSynchronize(s));
+    if (__builtin_popcount(bits & ~1u) <= kSlotCodes) {
+      uint64_t lut = 0;
+      uint32_t slot = 0, slot_code = 0;
+      for (uint32_t c = 1; c < 16; c++) {
+        if (!((bits >> c) & 1u)) continue;
+        slot++;
+        lut |= (uint64_t)slot << (4 * c);
+        slot_code |= c << (4 * (slot - 1));
+      }
+      db->slot_codes.ensure(acc + 64);
+      launch_code_slots(db->codes.as<uint8_t>(), acc, lut, db->slot_codes.as<uint8_t>(), s);
+      db->compact = true;
+      db->slot_code = slot_code;
+    }
+  }
   NGIA_HIP(hipStreamSynchronize(s));
   db->have_sig = false;
   db->rep_bands = 0;
@@ -235,6 +273,10 @@ inline AlignArgs align_args(const ngia_db* db, float thr) {
   a.code_off = db->code_off.as<uint64_t>();
   a.thr = thr;
   a.raw_lcs = db->ctx && db->ctx->raw_lcs ? 1u : 0u;
+  if (db->compact) {
+    a.slots = db->slot_codes.as<uint8_t>();
+    a.slot_code = db->slot_code;
+  }
   return a;
 }
 

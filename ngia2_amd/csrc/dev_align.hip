@@ -58,6 +58,56 @@ void launch_codes(const uint32_t* packed, const uint64_t* word_off, const uint64
   NGIA_HIP(hipGetLastError());
 }
 
+// Which codes occur: one bit per code, OR-reduced per block, then one atomic.
+__global__ __launch_bounds__(256) void k_code_set(const uint32_t* __restrict__ codes, uint64_t words,
+                                                  uint32_t* __restrict__ present) {
+  uint32_t bits = 0;
+  for (uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x; k < words;
+       k += (uint64_t)gridDim.x * 256) {
+    const uint32_t w = codes[k];
+    bits |= (1u << (w & 0xFu)) | (1u << ((w >> 8) & 0xFu)) | (1u << ((w >> 16) & 0xFu)) |
+            (1u << ((w >> 24) & 0xFu));
+  }
+  __shared__ uint32_t acc;
+  if (threadIdx.x == 0) acc = 0;
+  __syncthreads();
+  if (bits) atomicOr(&acc, bits);
+  __syncthreads();
+  if (threadIdx.x == 0 && acc) atomicOr(present, acc);
+}
+
+void launch_code_set(const uint8_t* codes, uint64_t bytes, uint32_t* present, hipStream_t stream) {
+  const uint64_t words = bytes / 4;  // the code area is a whole number of 32-byte blocks
+  if (words == 0) return;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((words + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_code_set, dim3(grid), dim3(256), 0, stream, (const uint32_t*)codes, words,
+                     present);
+  NGIA_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void k_code_slots(const uint32_t* __restrict__ codes, uint64_t words,
+                                                    uint64_t lut, uint32_t* __restrict__ slots) {
+  for (uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x; k < words;
+       k += (uint64_t)gridDim.x * 256) {
+    const uint32_t w = codes[k];
+    uint32_t o = 0;
+    #pragma unroll
+    for (int b = 0; b < 4; b++)
+      o |= (uint32_t)((lut >> (4 * ((w >> (8 * b)) & 0xFu))) & 0xFu) << (8 * b);
+    slots[k] = o;
+  }
+}
+
+void launch_code_slots(const uint8_t* codes, uint64_t bytes, uint64_t lut, uint8_t* slots,
+                       hipStream_t stream) {
+  const uint64_t words = bytes / 4;
+  if (words == 0) return;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((words + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_code_slots, dim3(grid), dim3(256), 0, stream, (const uint32_t*)codes, words,
+                     lut, (uint32_t*)slots);
+  NGIA_HIP(hipGetLastError());
+}
+
 // The 15 match masks of one rep word from its 4 bit planes: mask[k] has bit
 // r set iff residue r of the word has code k (buildGraph.hpp:220-225).
 __device__ inline void peq_masks(uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3,
@@ -175,17 +225,41 @@ struct WinPair {
   const uint32_t* tc;    // target byte codes, 8 dwords per block
   int32_t c, nw;
   uint32_t nb;
-  uint32_t* lds;         // [W][16][64] Peq slots, then [8][64] staged codes
+  uint32_t* lds;         // [W][NC][64] Peq slots, then [8][64] staged codes
   uint32_t lane, lane4;
   uint4 nc0, nc1;        // codes of the next block, loaded one block ahead
+  uint32_t slot_code;    // NC < 16: the code of slot s (1..NC-1) in bits 4(s-1)..
 };
 
-template <int W, int Q>
+// The masks of one entering rep word into its Peq slot: the 15 nonzero codes
+// (NC = 16), or the codes of slots 1..NC-1 of a compact alphabet.
+template <int NC>
+__device__ __forceinline__ void win_store_masks(uint32_t* slot, uint32_t lane, uint32_t p0,
+                                                uint32_t p1, uint32_t p2, uint32_t p3,
+                                                uint32_t slot_code) {
+  if constexpr (NC == 16) {
+    uint32_t m[16];
+    peq_masks(p0, p1, p2, p3, m);
+    #pragma unroll
+    for (int k = 1; k < 16; k++) slot[k * 64 + lane] = m[k];
+  } else {
+    #pragma unroll
+    for (int k = 1; k < NC; k++) {
+      const uint32_t code = (slot_code >> (4 * (k - 1))) & 0xFu;
+      const uint32_t x0 = (code & 1u) ? p0 : ~p0, x1 = (code & 2u) ? p1 : ~p1;
+      const uint32_t x2 = (code & 4u) ? p2 : ~p2, x3 = (code & 8u) ? p3 : ~p3;
+      slot[k * 64 + lane] = x0 & x1 & x2 & x3;
+    }
+  }
+}
+
+template <int W, int NC, int Q>
 __device__ __forceinline__ bool win_block(WinPair<W>& s, uint32_t cb0) {
+  constexpr int kSlot = NC * 64;  // dwords per Peq slot
   const uint32_t cb = cb0 + Q;
   if (cb >= s.nb) return false;
   uint32_t* const lds = s.lds;
-  uint32_t* const cst = lds + W * 1024;
+  uint32_t* const cst = lds + W * kSlot;
   const uint32_t lane = s.lane;
   // This block's codes arrived during the previous block; fetch the next
   // block's now (past the last block the read stays inside the buffer's
@@ -210,7 +284,8 @@ __device__ __forceinline__ bool win_block(WinPair<W>& s, uint32_t cb0) {
     for (int mm = 0; mm < 4; mm++) {
       // byte address of Peq[code][lane] = code*256 + lane*4: v_perm takes
       // byte 0 from lane4 (selector 0) and byte 1 from code byte mm of cw
-      // (selector 4+mm); bytes 2,3 are zero (selector 0x0C).
+      // (selector 4+mm); bytes 2,3 are zero (selector 0x0C).  With a compact
+      // alphabet the byte is the residue's slot.
       const uint32_t addr =
           __builtin_amdgcn_perm(cw, s.lane4, 0x0C0C0000u | ((4u + (uint32_t)mm) << 8));
       const uint32_t* pq = (const uint32_t*)((const char*)lds + addr);
@@ -219,43 +294,44 @@ __device__ __forceinline__ bool win_block(WinPair<W>& s, uint32_t cb0) {
       for (int j = 0; j < W; j++) {
         constexpr int kW = W;
         const int r = (Q + j) % kW;  // register and Peq slot of relative word j
-        s.rows[r] = lcs_step(s.rows[r], pq[r * 1024], cin, &cin);
+        s.rows[r] = lcs_step(s.rows[r], pq[r * kSlot], cin, &cin);
       }
     }
   }
   // Slide: relative word 0 (register Q) leaves, word cb + c enters there.
   s.lcs += 32u - __popc(s.rows[Q]);
   s.rows[Q] = 0xFFFFFFFFu;
-  uint32_t m[16];
   // Past the rep's last word the planes are 0, and so is every mask of a
   // nonzero code (each needs a set plane bit or a clear one with another set).
-  peq_masks(p0, p1, p2, p3, m);
-  #pragma unroll
-  for (int k = 1; k < 16; k++) lds[Q * 1024 + k * 64 + lane] = m[k];
+  win_store_masks<NC>(lds + Q * kSlot, lane, p0, p1, p2, p3, s.slot_code);
   return true;
 }
 
-template <int W, int Q>
+template <int W, int NC, int Q>
 __device__ __forceinline__ void win_round(WinPair<W>& s, uint32_t cb0) {
   if constexpr (Q < W) {
-    if (win_block<W, Q>(s, cb0)) win_round<W, Q + 1>(s, cb0);
+    if (win_block<W, NC, Q>(s, cb0)) win_round<W, NC, Q + 1>(s, cb0);
   }
 }
 
-template <int W>
+// NC = 16 indexes the Peq table by residue code; NC < 16 by the slot of a
+// compact alphabet (A.slots), which shrinks the table and raises occupancy.
+template <int W, int NC>
 __global__ __launch_bounds__(64) void k_align_win(AlignArgs A, const uint2* __restrict__ pairs,
                                                   const uint32_t* __restrict__ list,
                                                   const uint32_t* __restrict__ off_ptr,
                                                   const uint32_t* __restrict__ count_ptr,
                                                   uint32_t* __restrict__ work,
                                                   uint32_t* __restrict__ score) {
-  // Peq slots [W][16 codes][64 lanes] + staged target codes [8][64 lanes].
+  constexpr int kSlot = NC * 64;
+  // Peq slots [W][NC codes][64 lanes] + staged target codes [8][64 lanes].
   // Static so that its LDS address is a compile-time constant.
-  __shared__ uint32_t lds[W * 1024 + 8 * 64];
+  __shared__ uint32_t lds[W * kSlot + 8 * 64];
   const uint32_t lane = threadIdx.x;
   const uint32_t lane4 = lane * 4;
+  const uint8_t* const rcodes = NC == 16 ? A.codes : A.slots;
   #pragma unroll
-  for (int s = 0; s < W; s++) lds[s * 1024 + lane] = 0;  // code 0 never matches
+  for (int s = 0; s < W; s++) lds[s * kSlot + lane] = 0;  // code (slot) 0 never matches
   const uint32_t count = *count_ptr;
   const uint32_t* const lst = list + (off_ptr ? *off_ptr : 0u);
   for (;;) {
@@ -275,27 +351,25 @@ __global__ __launch_bounds__(64) void k_align_win(AlignArgs A, const uint2* __re
     const int32_t c = (int32_t)((bd.rsft + 31) / 32);
     const int32_t nw = (int32_t)((R + 31) / 32);
     const uint32_t nb = (T + 31) / 32;
-    const uint32_t* tc = (const uint32_t*)(A.codes + A.code_off[pr.y]);
+    const uint32_t* tc = (const uint32_t*)(rcodes + A.code_off[pr.y]);
     // Window at block 0: relative word j <-> absolute j - a, register j, slot j.
+    // Words outside the record get the masks of all-zero planes: 0 for every
+    // nonzero code.
     uint32_t rows[W];
     #pragma unroll
     for (int j = 0; j < W; j++) {
       rows[j] = 0xFFFFFFFFu;
       const int32_t w = j - a;
-      uint32_t m[16];
+      uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
       if (w >= 0 && w < nw) {
         const uint32_t* pl = rr + 1 + 4 * w;
-        peq_masks(pl[0], pl[1], pl[2], pl[3], m);
-      } else {
-        #pragma unroll
-        for (int k = 0; k < 16; k++) m[k] = 0;
+        p0 = pl[0]; p1 = pl[1]; p2 = pl[2]; p3 = pl[3];
       }
-      #pragma unroll
-      for (int k = 1; k < 16; k++) lds[j * 1024 + k * 64 + lane] = m[k];
+      win_store_masks<NC>(lds + j * kSlot, lane, p0, p1, p2, p3, A.slot_code);
     }
     WinPair<W> st{rows, 0u, rr, tc, c, nw, nb, lds, lane, lane4,
-                  *(const uint4*)tc, *(const uint4*)(tc + 4)};
-    for (uint32_t cb0 = 0; cb0 < nb; cb0 += W) win_round<W, 0>(st, cb0);
+                  *(const uint4*)tc, *(const uint4*)(tc + 4), A.slot_code};
+    for (uint32_t cb0 = 0; cb0 < nb; cb0 += W) win_round<W, NC, 0>(st, cb0);
     #pragma unroll
     for (int j = 0; j < W; j++) st.lcs += 32u - __popc(rows[j]);
     score[p] = pair_value(A, st.lcs, R, T);
@@ -492,13 +566,19 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
 }
 
 // ------------------------------------------------ launchers for the bins
-size_t win_lds_bytes(int W) { return (size_t)W * 4096 + 8 * 256; }
+size_t win_lds_bytes(int W, bool compact) {
+  return (size_t)W * (compact ? kSlotCodes + 1 : 16) * 256 + 8 * 256;
+}
 size_t wave_lds_bytes(int K) { return (size_t)16 * 64 * K * 4; }
 
 #define NGIA_WIN_CASE(Wv)                                                                  \
   case Wv:                                                                                 \
-    hipLaunchKernelGGL(k_align_win<Wv>, dim3(blocks), dim3(64), 0, stream,                 \
-                       a, pairs, list, off_ptr, count_ptr, work, score);                   \
+    if (a.slots)                                                                           \
+      hipLaunchKernelGGL((k_align_win<Wv, kSlotCodes + 1>), dim3(blocks), dim3(64), 0,     \
+                         stream, a, pairs, list, off_ptr, count_ptr, work, score);         \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_align_win<Wv, 16>), dim3(blocks), dim3(64), 0, stream, a,      \
+                         pairs, list, off_ptr, count_ptr, work, score);                    \
     break;
 
 void launch_align_win(int W, const AlignArgs& a, const uint2* pairs, const uint32_t* list,

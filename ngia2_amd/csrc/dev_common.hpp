@@ -68,6 +68,11 @@ void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch
 
 // Byte-per-residue codes (code of residue j at byte code_off[i] + j, padding
 // to a multiple of 32 is code 0) built from the packed planes.
+// Bit c of *present |= 1 when code c occurs among the first `bytes` codes.
+void launch_code_set(const uint8_t* codes, uint64_t bytes, uint32_t* present, hipStream_t stream);
+// slots[k] = slot of codes[k] under lut (slot of code c in bits 4c..4c+3).
+void launch_code_slots(const uint8_t* codes, uint64_t bytes, uint64_t lut, uint8_t* slots,
+                       hipStream_t stream);
 void launch_codes(const uint32_t* packed, const uint64_t* word_off, const uint64_t* code_off,
                   uint32_t n, uint8_t* codes, hipStream_t stream);
 
@@ -85,7 +90,15 @@ struct AlignArgs {
   const uint64_t* code_off;
   float thr;
   uint32_t raw_lcs = 0;  // nonzero: store the LCS length instead of the score
+  // Compact alphabet (a database whose residues use at most kSlotCodes
+  // nonzero codes, DNA among them): slots[] holds each residue's slot
+  // (0 for code 0, 1..kSlotCodes for the codes present), laid out as codes[];
+  // slot_code holds the code of slot s in bits 4(s-1)..4s-1.  nullptr: the
+  // full 16-code alphabet.
+  const uint8_t* slots = nullptr;
+  uint32_t slot_code = 0;
 };
+constexpr int kSlotCodes = 5;
 // What an alignment kernel stores for a pair it aligned: the score, or the
 // LCS length itself when raw_lcs is set (a check of the recurrence that does
 // not go through the score's integer rounding and threshold).
@@ -103,7 +116,7 @@ uint64_t align_plain_scratch_words(uint32_t nw_max);
 // Length-binned forms over a pair list (pairs[list[*off_ptr + i]], i <
 // *count_ptr); work is a zeroed work-queue counter; blocks is the persistent
 // grid.
-size_t win_lds_bytes(int W);
+size_t win_lds_bytes(int W, bool compact = false);
 size_t wave_lds_bytes(int K);
 void launch_align_win(int W, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
                       const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work, uint32_t* score,

@@ -506,6 +506,46 @@ def test_dna_graph(ctx, dnadb, identity, mode):
     assert len(want[identity]) > 500
 
 
+@pytest.mark.parametrize("alpha", [b"ACGT", b"ACGTN", b"ACGTX", b"ACD"])
+def test_compact_alphabet_window_kernels(ctx, tmp_path, alpha):
+    """A database whose residues use at most five nonzero codes (DNA; DNA with
+    N, code 11; with X, code 0; a three-letter protein alphabet) is aligned by the
+    window kernels with a table indexed by slot: every pair scores as with the
+    16-code table (a context with NGIA_FULL_ALPHABET=1) and as the oracle."""
+    fa, dbp = str(tmp_path / "a.fa"), str(tmp_path / "a.bin")
+    rng = np.random.default_rng(21)
+    lengths = [40, 90, 200, 333, 480, 700, 1000, 1500, 2100, 3000]
+    family_fasta(fa, rng, lengths, members=6, sub=0.04, indel=0.01, alpha=alpha)
+    assert orc.makedb(fa, dbp) == 0
+    d = orc.read_db_numpy(dbp)
+    os.environ["NGIA_FULL_ALPHABET"] = "1"
+    try:
+        full_ctx = ng.Context(0)
+    finally:
+        del os.environ["NGIA_FULL_ALPHABET"]
+    with full_ctx:
+        for identity in (50, 80, 90, 97):
+            say("compact alphabet", alpha, "identity", identity)
+            got = []
+            for c in (ctx, full_ctx):
+                db = ng.Database(c, d["packed"], d["word_off"], d["lengths"])
+                db.set_reps(d["rep"], 64)
+                db.graph(identity, 64, ng.GRAPH_BATCHED)
+                got.append((db.pair_scores(), db.edges(),
+                            sum(b["pairs"] for b in db.bin_stats() if b["kind"] == "window")))
+                db.close()
+            (keys, sc), edges, win = got[0]
+            np.testing.assert_array_equal(keys, got[1][0][0])
+            np.testing.assert_array_equal(sc, got[1][0][1])
+            np.testing.assert_array_equal(edges, got[1][1])
+            assert win > 0
+            jobs = np.zeros((len(keys), 3), dtype=np.uint32)
+            jobs[:, 0] = keys >> np.uint64(32)
+            jobs[:, 1] = keys & np.uint64(0xFFFFFFFF)
+            thr = float(np.float32(identity) / np.float32(100))
+            np.testing.assert_array_equal(sc, orc.align_jobs(d["packed"], d["word_off"], jobs, thr)[:, 2])
+
+
 def test_sketch_past_2gib_of_codes(ctx):
     """Sequences whose byte codes start beyond 2^31 bytes of the code area
     (1.6M DNA sequences of ~1400 nt): their signatures match the oracle."""
