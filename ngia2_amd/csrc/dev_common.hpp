@@ -61,7 +61,9 @@ __host__ __device__ inline uint32_t lcs_score(uint32_t lcs, uint32_t R, uint32_t
 
 // Band-major bucketing of n x 64 signatures into 64 x n representative rows.
 // scratch holds bucket_scratch_bytes(n, force_wide) bytes; force_wide selects
-// the 38-bit-key form that large n needs anyway.
+// the 38-bit-key full sort that large n needs anyway.  Below that, n up to
+// 2.8M takes the partitioned form, which synchronises the stream once (to
+// learn whether a partition overflowed and the full sort must run instead).
 size_t bucket_scratch_bytes(uint32_t n, bool force_wide = false);
 void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch,
                    hipStream_t stream, bool force_wide = false);

@@ -10,11 +10,13 @@
 // a wave-uniform term xor a per-lane constant, because a logical right shift
 // distributes over xor.  Output is sequence-major: lane k of sequence i
 // writes sig[64*i + k], one 256-byte coalesced store per wavefront.
+#include <algorithm>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
 
 #include "dev_common.hpp"
+#include "dev_graph.hpp"
 
 namespace ngia {
 
@@ -178,11 +180,144 @@ __global__ __launch_bounds__(256) void k_bucket_rep_wide(const uint64_t* __restr
   if (keys[p - 1] == k) rep[(k >> 32) * n + vals[p]] = vals[p - 1];
 }
 
+// ------------------------------------------------ partitioned bucketing
+// Only equality of signatures within a band matters, so the full sort above
+// is replaced, for n <= kPartMaxN, by a partition and a sort in LDS:
+//  1. every (band b, record i) gets h = mix(sig), a bijection of the 32-bit
+//     signature that spreads its bits (minhash values crowd near 0), the
+//     16-bit key (b << 10 | top 10 bits of h) and the value (h << 32 | i);
+//  2. a stable radix sort on the 16-bit key (two passes) groups each band's
+//     elements into 1024 partitions, each still in ascending i;
+//  3. one block per partition sorts it by h in LDS (stable), and every
+//     element equal to its predecessor gets rep[b][i] = that predecessor's i
+//     -- the element before it in the reference's stable (sig, i) order.
+// A partition larger than a block's capacity (heavily repeated signatures)
+// is counted; the host then redoes the whole bucketing with the full sort.
+constexpr uint32_t kPartBits = 10;
+constexpr uint32_t kPartMaxN = 2800000;  // records: partitions average n/1024 <= ~2734
+
+__device__ inline uint32_t sig_mix(uint32_t x) {
+  x *= 0x9E3779B1u;  // odd multiplier: a bijection mod 2^32
+  x ^= x >> 16;      // xor with a right shift of itself: a bijection
+  x *= 0x85EBCA6Bu;
+  x ^= x >> 13;
+  return x;
+}
+
+__global__ __launch_bounds__(256) void k_part_keys(const uint32_t* __restrict__ sig, uint32_t n,
+                                                   uint16_t* __restrict__ keys,
+                                                   uint64_t* __restrict__ vals) {
+  // 64 records x 64 bands through LDS, as in k_bucket_keys
+  __shared__ uint32_t tile[64][65];
+  const uint32_t i0 = blockIdx.x * 64;
+  const uint32_t t = threadIdx.x;
+  for (uint32_t e = t; e < 64 * 64; e += 256) {
+    const uint32_t si = e >> 6, b = e & 63;
+    tile[si][b] = (i0 + si < n) ? sig[(uint64_t)(i0 + si) * 64 + b] : 0u;
+  }
+  __syncthreads();
+  for (uint32_t e = t; e < 64 * 64; e += 256) {
+    const uint32_t b = e >> 6, si = e & 63;
+    const uint32_t i = i0 + si;
+    if (i < n) {
+      const uint32_t h = sig_mix(tile[si][b]);
+      keys[(uint64_t)b * n + i] = (uint16_t)((b << kPartBits) | (h >> (32 - kPartBits)));
+      vals[(uint64_t)b * n + i] = ((uint64_t)h << 32) | i;
+    }
+  }
+}
+
+// start[k], end[k]: the sorted range of partition k (left 0, 0 when empty).
+__global__ __launch_bounds__(256) void k_part_bounds(const uint16_t* __restrict__ keys, uint64_t m,
+                                                     uint32_t* __restrict__ start,
+                                                     uint32_t* __restrict__ end) {
+  for (uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x; p < m;
+       p += (uint64_t)gridDim.x * 256) {
+    const uint32_t k = keys[p];
+    if (p == 0 || keys[p - 1] != k) start[k] = (uint32_t)p;
+    if (p + 1 == m || keys[p + 1] != k) end[k] = (uint32_t)(p + 1);
+  }
+}
+
+template <int ITEMS>
+__global__ __launch_bounds__(256) void k_part_local(const uint64_t* __restrict__ vals,
+                                                    const uint32_t* __restrict__ start,
+                                                    const uint32_t* __restrict__ end, uint32_t n,
+                                                    uint32_t* __restrict__ rep,
+                                                    uint32_t* __restrict__ overflow) {
+  constexpr uint32_t kCap = 256 * ITEMS;
+  using Sort = rocprim::block_radix_sort<uint32_t, 256, ITEMS, uint32_t>;
+  __shared__ typename Sort::storage_type storage;
+  __shared__ uint32_t sk[kCap], sv[kCap];
+  const uint32_t part = blockIdx.x;
+  const uint32_t s = start[part], cnt = end[part] - s;
+  if (cnt < 2) return;  // a lone element is a bucket head: its row entry stays i
+  if (cnt > kCap) {
+    if (threadIdx.x == 0) atomicAdd(overflow, 1u);
+    return;
+  }
+  uint32_t key[ITEMS], val[ITEMS];
+  #pragma unroll
+  for (int j = 0; j < ITEMS; j++) {
+    const uint32_t q = threadIdx.x * ITEMS + j;  // blocked: input order = ascending i
+    if (q < cnt) {
+      const uint64_t v = vals[s + q];
+      key[j] = (uint32_t)(v >> 32);
+      val[j] = (uint32_t)v;
+    } else {  // padding sorts after every element (same top bits, stable)
+      key[j] = 0xFFFFFFFFu;
+      val[j] = 0xFFFFFFFFu;
+    }
+  }
+  // The partition shares the top kPartBits of h: sort the rest, stably.
+  Sort().sort(key, val, storage, 0, 32 - kPartBits);
+  #pragma unroll
+  for (int j = 0; j < ITEMS; j++) {
+    sk[threadIdx.x * ITEMS + j] = key[j];
+    sv[threadIdx.x * ITEMS + j] = val[j];
+  }
+  __syncthreads();
+  const uint64_t row = (uint64_t)(part >> kPartBits) * n;
+  for (uint32_t q = threadIdx.x + 1; q < cnt; q += 256)
+    if (sk[q] == sk[q - 1]) rep[row + sv[q]] = sv[q - 1];
+}
+
 namespace {
 struct BucketLayout {
   bool wide;
   size_t keys_in, keys_out, vals_in, vals_out, temp, temp_bytes, total;
 };
+// partitioned path: 16-bit keys, 64-bit values, the partition ranges and the
+// overflow count
+struct PartLayout {
+  bool use;
+  int items;
+  size_t keys_in, keys_out, vals_in, vals_out, start, end, overflow, temp, temp_bytes, total;
+};
+PartLayout part_layout(uint32_t n, bool force_wide) {
+  PartLayout L{};
+  L.use = !force_wide && n <= kPartMaxN;
+  if (!L.use) return L;
+  const uint64_t m = (uint64_t)n * 64;
+  // capacity at least twice the average partition, n / 1024
+  L.items = n <= 1400000u ? 8 : 16;
+  size_t temp_bytes = 0;
+  (void)rocprim::radix_sort_pairs(nullptr, temp_bytes, (uint16_t*)nullptr, (uint16_t*)nullptr,
+                                  (uint64_t*)nullptr, (uint64_t*)nullptr, (size_t)m, 0, 16);
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t parts = (size_t)64 << kPartBits;
+  L.keys_in = 0;
+  L.keys_out = L.keys_in + al(2 * m);
+  L.vals_in = L.keys_out + al(2 * m);
+  L.vals_out = L.vals_in + al(8 * m);
+  L.start = L.vals_out + al(8 * m);
+  L.end = L.start + al(4 * parts);
+  L.overflow = L.end + al(4 * parts);
+  L.temp = L.overflow + 256;
+  L.temp_bytes = temp_bytes;
+  L.total = L.temp + al(temp_bytes);
+  return L;
+}
 BucketLayout bucket_layout(uint32_t n, bool force_wide) {
   const uint64_t m = (uint64_t)n * 64;
   BucketLayout L;
@@ -208,8 +343,47 @@ BucketLayout bucket_layout(uint32_t n, bool force_wide) {
 }  // namespace
 
 size_t bucket_scratch_bytes(uint32_t n, bool force_wide) {
-  return bucket_layout(n, force_wide).total;
+  return std::max(bucket_layout(n, force_wide).total, part_layout(n, force_wide).total);
 }
+
+namespace {
+// The partitioned bucketing; returns false when a partition overflowed (rep
+// then holds a partial result and the caller redoes it with the full sort).
+bool bucket_partitioned(const PartLayout& L, const uint32_t* sig, uint32_t n, uint32_t* rep,
+                        char* s, hipStream_t stream) {
+  const uint64_t m = (uint64_t)n * 64;
+  const uint32_t parts = 64u << kPartBits;
+  uint16_t* keys_in = (uint16_t*)(s + L.keys_in);
+  uint16_t* keys_out = (uint16_t*)(s + L.keys_out);
+  uint64_t* vals_in = (uint64_t*)(s + L.vals_in);
+  uint64_t* vals_out = (uint64_t*)(s + L.vals_out);
+  uint32_t* start = (uint32_t*)(s + L.start);
+  uint32_t* end = (uint32_t*)(s + L.end);
+  uint32_t* overflow = (uint32_t*)(s + L.overflow);
+  hipLaunchKernelGGL(k_part_keys, dim3((n + 63) / 64), dim3(256), 0, stream, sig, n, keys_in,
+                     vals_in);
+  NGIA_HIP(hipGetLastError());
+  size_t tb = L.temp_bytes;
+  NGIA_HIP(rocprim::radix_sort_pairs(s + L.temp, tb, keys_in, keys_out, vals_in, vals_out, (size_t)m,
+                                     0, 16, stream));
+  fill_u32(start, 0, parts, stream);
+  fill_u32(end, 0, parts, stream);
+  fill_u32(overflow, 0, 1, stream);
+  hipLaunchKernelGGL(k_part_bounds, dim3(std::min<uint64_t>((m + 255) / 256, 16384)), dim3(256), 0,
+                     stream, keys_out, m, start, end);
+  if (L.items == 8)
+    hipLaunchKernelGGL(k_part_local<8>, dim3(parts), dim3(256), 0, stream, vals_out, start, end, n,
+                       rep, overflow);
+  else
+    hipLaunchKernelGGL(k_part_local<16>, dim3(parts), dim3(256), 0, stream, vals_out, start, end, n,
+                       rep, overflow);
+  NGIA_HIP(hipGetLastError());
+  uint32_t over = 0;
+  NGIA_HIP(hipMemcpyAsync(&over, overflow, 4, hipMemcpyDeviceToHost, stream));
+  NGIA_HIP(hipStreamSynchronize(stream));
+  return over == 0;
+}
+}  // namespace
 
 void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch,
                    hipStream_t stream, bool force_wide) {
@@ -222,6 +396,13 @@ void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch
   size_t tb = L.temp_bytes;
   hipLaunchKernelGGL(k_rep_identity, dim3(grid_for(n, 256), 64), dim3(256), 0, stream, n, rep);
   NGIA_HIP(hipGetLastError());
+  const PartLayout P = part_layout(n, force_wide);
+  if (P.use) {
+    if (bucket_partitioned(P, sig, n, rep, s, stream)) return;
+    // a partition overflowed: start again from the identity rows, full sort
+    hipLaunchKernelGGL(k_rep_identity, dim3(grid_for(n, 256), 64), dim3(256), 0, stream, n, rep);
+    NGIA_HIP(hipGetLastError());
+  }
   if (!L.wide) {
     uint32_t* keys_in = (uint32_t*)(s + L.keys_in);
     uint32_t* keys_out = (uint32_t*)(s + L.keys_out);

@@ -102,6 +102,47 @@ def test_bucket_collisions(ctx):
     np.testing.assert_array_equal(ctx.bucket(sig), orc.bucket(sig))
 
 
+def test_bucket_partition_collisions(ctx):
+    """Many equal signatures inside the partitions' capacity (the sort in LDS
+    orders up to ~40 copies of each value per band), and a band where every
+    record has the same signature, so that one partition overflows and the
+    bucketing is redone with the full sort."""
+    rng = np.random.default_rng(8)
+    sig = rng.integers(0, 50, size=(2000, 64)).astype(np.uint32) * np.uint32(0x01000193)
+    say("bucket partition collisions", sig.shape)
+    np.testing.assert_array_equal(ctx.bucket(sig), orc.bucket(sig))
+    over = rng.integers(0, 1 << 32, size=(5000, 64), dtype=np.uint64).astype(np.uint32)
+    over[:, 3] = 12345
+    say("bucket partition overflow", over.shape)
+    np.testing.assert_array_equal(ctx.bucket(over), orc.bucket(over))
+
+
+def family_sigs(rng, n, fam=20, keep=0.7):
+    """Signature rows of n records in families of `fam`: each entry is the
+    family's value with probability `keep`, else random."""
+    base = rng.integers(0, 1 << 32, size=((n + fam - 1) // fam, 64), dtype=np.uint64).astype(np.uint32)
+    sig = base[np.arange(n) // fam]
+    noise = rng.integers(0, 1 << 32, size=(n, 64), dtype=np.uint64).astype(np.uint32)
+    return np.where(rng.random((n, 64)) < keep, sig, noise)
+
+
+@pytest.mark.parametrize("n", [1000000, 1500000])
+def test_bucket_partitioned_large(ctx, n):
+    """At a million records and more (both block capacities of the partitioned
+    form) the rows equal those of the 38-bit full sort."""
+    sig = family_sigs(np.random.default_rng(n), n)
+    say("bucket partitioned vs full sort", n)
+    got = ctx.bucket(sig)
+    os.environ["NGIA_BUCKET_WIDE"] = "1"
+    try:
+        with ng.Context(0) as c2:
+            want = c2.bucket(sig)
+    finally:
+        del os.environ["NGIA_BUCKET_WIDE"]
+    np.testing.assert_array_equal(got, want)
+    assert (got != np.arange(n, dtype=np.uint32)[None, :]).mean() > 0.5
+
+
 @pytest.mark.parametrize("identity", [1, 50, 70, 90, 97, 99])
 def test_align_random_pairs(ctx, c1db, identity):
     _, _, d = c1db
