@@ -12,7 +12,6 @@
 
 #include <algorithm>
 #include <chrono>
-#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -131,9 +130,10 @@ void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ban
   Counters* dc = db->counters.as<Counters>();
   const uint32_t* len = db->lengths.as<uint32_t>();
   fill_u32(db->counters.as<uint32_t>(), 0, sizeof(Counters) / 4, s);
-  pair_count(db->rep.as<uint32_t>(), len, n, bands, thr, db->cnt.as<uint32_t>(), &dc->total_cand, s);
+  pair_count(db->rep.as<uint32_t>(), len, n, bands, thr, db->cnt.as<uint32_t>(),
+             db->keep.as<uint64_t>(), &dc->total_cand, s);
   scan_u32(db->cnt.as<uint32_t>(), n, db->off.as<uint64_t>(), db->sums.as<uint64_t>(), s);
-  pair_emit(db->rep.as<uint32_t>(), len, n, bands, thr, db->off.as<uint64_t>(),
+  pair_emit(db->rep.as<uint32_t>(), n, bands, db->keep.as<uint64_t>(), db->off.as<uint64_t>(),
             db->pairs.as<uint2>(), s);
   const uint64_t* npairs = db->off.as<uint64_t>() + n;
   const uint32_t grid = persistent_blocks(ctx, 0) * 4;
@@ -207,8 +207,9 @@ void ensure_batched_space(ngia_db* db, uint32_t bands) {
   const uint32_t n = db->n;
   const uint64_t cap = std::max<uint64_t>((uint64_t)n * bands, 1);  // distinct <= candidates <= n*bands
   db->counters.ensure(sizeof(Counters));
-  db->cnt.ensure(4ull * n);
   db->off.ensure(8ull * (n + 1));
+  db->cnt.ensure(4ull * n);
+  db->keep.ensure(8ull * n);
   db->sums.ensure(8 * scan_scratch_words(n));
   db->pairs.ensure(8 * cap);
   db->score.ensure(4 * cap);
@@ -233,7 +234,7 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   // of it changed (a grown buffer moves, a new tuning changes the routing).
   std::vector<uintptr_t> key;
   for (const DevBuf* b : {&db->packed, &db->word_off, &db->lengths, &db->codes, &db->code_off,
-                          &db->rep, &db->counters, &db->cnt, &db->off, &db->sums, &db->pairs,
+                          &db->rep, &db->counters, &db->cnt, &db->keep, &db->off, &db->sums, &db->pairs,
                           &db->score, &db->bin_of, &db->list, &db->plain_jobs, &db->plain_scratch,
                           &db->ekeys, &db->escores})
     key.push_back((uintptr_t)b->p);

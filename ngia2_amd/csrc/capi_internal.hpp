@@ -118,7 +118,8 @@ struct ngia_db {
   uint32_t slot_code = 0;
   // graph workspace
   ngia::DevBuf flags, pos, sums, jobs, ekeys, escores, ekeys_tmp, escores_tmp, fin_temp;
-  ngia::DevBuf counters, cnt, off, pairs, score, bin_of, list, plain_scratch, plain_jobs, bucket_scratch;
+  ngia::DevBuf counters, cnt, keep, off, pairs, score, bin_of, list, plain_scratch, plain_jobs,
+      bucket_scratch;
   uint64_t n_edges = 0, n_pairs = 0;
   // the captured batched round; g_key holds the buffer addresses and the
   // routing it was captured with

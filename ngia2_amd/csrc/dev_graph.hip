@@ -5,6 +5,7 @@
 // bands, a persistent grid) and reads data-dependent counts from device
 // memory, so a whole round can be captured in a hipGraph.  Counters are
 // cleared with a fill kernel, never with hipMemsetAsync.
+#include <algorithm>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -212,37 +213,57 @@ void fast_apply(const uint32_t* jobs, uint32_t count, uint32_t* parent, hipStrea
 }
 
 // ------------------------------------------------------------- batched form
-// A block takes 64 consecutive targets: the 64 x 64 tile rep[b][i0..i0+63]
-// is read row by row (coalesced) into LDS, then each wavefront handles 16
-// targets, lane b holding rep[b][i].  The distinct representatives passing
-// the candidate predicate are kept in order of their first band.  Count pass
-// (pairs == nullptr) writes cnt[i] and adds the candidate count with
-// duplicates to *total; emit pass writes the pairs at off[i].
-__global__ __launch_bounds__(256) void k_pair_scan(const uint32_t* __restrict__ rep,
-                                                   const uint32_t* __restrict__ len, uint32_t n,
-                                                   uint32_t bands, float thr,
-                                                   uint32_t* __restrict__ cnt,
-                                                   unsigned long long* __restrict__ total,
-                                                   const uint64_t* __restrict__ off,
-                                                   uint2* __restrict__ pairs) {
-  __shared__ uint32_t tile[64][65];
-  __shared__ unsigned long long btotal;
-  const uint32_t i0 = blockIdx.x * 64;
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t wave = threadIdx.x >> 6;
-  if (threadIdx.x == 0) btotal = 0;
+// Count pass: a block takes 64 consecutive targets.  The 64 x 64 tile
+// rep[b][i0..i0+63] is read row by row (coalesced) into LDS; each wavefront
+// handles 16 targets, lane b holding rep[b][i].  The lengths of all 16
+// targets and their representatives are gathered before the first is tested,
+// so the wave waits on memory once, not 16 times.  The distinct
+// representatives passing the candidate predicate are kept in order of their
+// first band: keep[i] has bit b set for the band that contributes each kept
+// representative, cnt[i] = its popcount, and *total gains the candidate
+// count with duplicates.
+__device__ inline void load_rep_tile(const uint32_t* __restrict__ rep, uint32_t n, uint32_t bands,
+                                     uint32_t i0, uint32_t (&tile)[64][65]) {
   for (uint32_t e = threadIdx.x; e < 64 * 64; e += 256) {
     const uint32_t b = e >> 6, t = e & 63;
     tile[b][t] = (b < bands && i0 + t < n) ? rep[(uint64_t)b * n + i0 + t] : 0u;
   }
+}
+
+__global__ __launch_bounds__(256) void k_pair_count(const uint32_t* __restrict__ rep,
+                                                    const uint32_t* __restrict__ len, uint32_t n,
+                                                    uint32_t bands, float thr,
+                                                    uint32_t* __restrict__ cnt,
+                                                    uint64_t* __restrict__ keep_out,
+                                                    unsigned long long* __restrict__ total) {
+  __shared__ uint32_t tile[64][65];
+  __shared__ unsigned long long btotal;
+  const uint32_t i0 = blockIdx.x * 64;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (threadIdx.x == 0) btotal = 0;
+  load_rep_tile(rep, n, bands, i0, tile);
   __syncthreads();
+  uint32_t lr[16], li[16];
+  uint32_t need = 0;  // bit tt: lane's band has a representative other than the target
+  #pragma unroll
+  for (int tt = 0; tt < 16; tt++) {
+    const uint32_t t = wave * 16 + tt, i = i0 + t;
+    const uint32_t r = tile[lane][t];
+    const bool nd = lane < bands && i < n && r != i;
+    lr[tt] = len[nd ? r : 0u];  // unconditional: rows hold indices < n
+    li[tt] = len[i < n ? i : 0u];
+    need |= (uint32_t)nd << tt;
+  }
   uint32_t wtotal = 0;
-  for (uint32_t tt = 0; tt < 16; tt++) {
+  #pragma unroll
+  for (int tt = 0; tt < 16; tt++) {
     const uint32_t t = wave * 16 + tt;
     const uint32_t i = i0 + t;
     if (i >= n) break;
     const uint32_t r = tile[lane][t];
-    const bool valid = lane < bands && is_job(r, i, len, thr);
+    // is_job: r != i and (float)len[i] >= (float)len[r] * thr
+    const bool valid = ((need >> tt) & 1u) && (float)li[tt] >= (float)lr[tt] * thr;
     const uint64_t vm = __builtin_amdgcn_ballot_w64(valid);
     uint64_t mask = vm, keep = 0;
     while (mask) {
@@ -251,32 +272,54 @@ __global__ __launch_bounds__(256) void k_pair_scan(const uint32_t* __restrict__ 
       keep |= 1ull << b0;
       mask &= ~__builtin_amdgcn_ballot_w64(valid && r == r0);
     }
-    if (!pairs) {
-      if (lane == 0) cnt[i] = (uint32_t)__builtin_popcountll(keep);
-      wtotal += (uint32_t)__builtin_popcountll(vm);
-    } else if ((keep >> lane) & 1ull) {
-      const uint32_t rank = (uint32_t)__builtin_popcountll(keep & ((1ull << lane) - 1ull));
-      pairs[off[i] + rank] = make_uint2(r, i);
+    if (lane == 0) {
+      cnt[i] = (uint32_t)__builtin_popcountll(keep);
+      keep_out[i] = keep;
     }
+    wtotal += (uint32_t)__builtin_popcountll(vm);
   }
-  if (!pairs) {
-    if (lane == 0 && wtotal) atomicAdd(&btotal, (unsigned long long)wtotal);
-    __syncthreads();
-    if (threadIdx.x == 0 && btotal) atomicAdd(total, btotal);
+  if (lane == 0 && wtotal) atomicAdd(&btotal, (unsigned long long)wtotal);
+  __syncthreads();
+  if (threadIdx.x == 0 && btotal) atomicAdd(total, btotal);
+}
+
+// Emit pass: the same tile; target i's kept representatives, in band order,
+// go to pairs[off[i] ..].
+__global__ __launch_bounds__(256) void k_pair_emit(const uint32_t* __restrict__ rep, uint32_t n,
+                                                   uint32_t bands,
+                                                   const uint64_t* __restrict__ keep_in,
+                                                   const uint64_t* __restrict__ off,
+                                                   uint2* __restrict__ pairs) {
+  __shared__ uint32_t tile[64][65];
+  const uint32_t i0 = blockIdx.x * 64;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  load_rep_tile(rep, n, bands, i0, tile);
+  __syncthreads();
+  #pragma unroll
+  for (int tt = 0; tt < 16; tt++) {
+    const uint32_t t = wave * 16 + tt;
+    const uint32_t i = i0 + t;
+    if (i >= n) break;
+    const uint64_t keep = keep_in[i];
+    if ((keep >> lane) & 1ull) {
+      const uint32_t rank = (uint32_t)__builtin_popcountll(keep & ((1ull << lane) - 1ull));
+      pairs[off[i] + rank] = make_uint2(tile[lane][t], i);
+    }
   }
 }
 
 void pair_count(const uint32_t* rep, const uint32_t* len, uint32_t n, uint32_t bands, float thr,
-                uint32_t* cnt, unsigned long long* total, hipStream_t s) {
-  hipLaunchKernelGGL(k_pair_scan, dim3((n + 63) / 64), dim3(256), 0, s, rep, len, n, bands, thr,
-                     cnt, total, (const uint64_t*)nullptr, (uint2*)nullptr);
+                uint32_t* cnt, uint64_t* keep, unsigned long long* total, hipStream_t s) {
+  hipLaunchKernelGGL(k_pair_count, dim3((n + 63) / 64), dim3(256), 0, s, rep, len, n, bands, thr,
+                     cnt, keep, total);
   NGIA_HIP(hipGetLastError());
 }
 
-void pair_emit(const uint32_t* rep, const uint32_t* len, uint32_t n, uint32_t bands, float thr,
+void pair_emit(const uint32_t* rep, uint32_t n, uint32_t bands, const uint64_t* keep,
                const uint64_t* off, uint2* pairs, hipStream_t s) {
-  hipLaunchKernelGGL(k_pair_scan, dim3((n + 63) / 64), dim3(256), 0, s, rep, len, n, bands, thr,
-                     (uint32_t*)nullptr, (unsigned long long*)nullptr, off, pairs);
+  hipLaunchKernelGGL(k_pair_emit, dim3((n + 63) / 64), dim3(256), 0, s, rep, n, bands, keep, off,
+                     pairs);
   NGIA_HIP(hipGetLastError());
 }
 
@@ -405,8 +448,11 @@ void bin_pairs(const uint2* pairs, const uint64_t* npairs, const uint32_t* len, 
                uint32_t* bin_of, uint32_t* bin_cnt, uint32_t* bin_off, uint32_t* cursor,
                uint32_t* list, unsigned long long* bin_tiles, uint32_t* score, uint32_t kmax,
                uint32_t* remap, const RouteTable& route, uint32_t grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_count, dim3(grid), dim3(256), 0, s, pairs, npairs, len, thr, kmax, bin_of,
-                     bin_cnt, bin_tiles, score);
+  // A sixteenth of the grid: every block ends with one global atomic per
+  // nonempty bin, and a few bins take nearly all pairs, so those counters
+  // see one atomic per block.
+  hipLaunchKernelGGL(k_bin_count, dim3(std::max(grid / 16, 1u)), dim3(256), 0, s, pairs, npairs, len,
+                     thr, kmax, bin_of, bin_cnt, bin_tiles, score);
   hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, bin_cnt, bin_off, cursor, remap,
                      bin_tiles, route);
   hipLaunchKernelGGL(k_bin_scatter, dim3(grid), dim3(256), 0, s, npairs, bin_of, remap, bin_off,

@@ -239,47 +239,74 @@ __global__ __launch_bounds__(256) void k_part_bounds(const uint16_t* __restrict_
   }
 }
 
+// One partition sorted in LDS by the low 32 - kPartBits bits of h (stable,
+// so equal values stay in ascending i), ITEMS elements per thread in the
+// blocked arrangement.  The predecessor of a thread's item j is its item
+// j - 1, or for j = 0 the last item of the thread before it.
 template <int ITEMS>
-__global__ __launch_bounds__(256) void k_part_local(const uint64_t* __restrict__ vals,
-                                                    const uint32_t* __restrict__ start,
-                                                    const uint32_t* __restrict__ end, uint32_t n,
-                                                    uint32_t* __restrict__ rep,
-                                                    uint32_t* __restrict__ overflow) {
-  constexpr uint32_t kCap = 256 * ITEMS;
+__device__ inline void part_sort_tier(const uint64_t* __restrict__ v, uint32_t cnt, uint64_t row,
+                                      uint32_t* __restrict__ rep, void* storage_raw,
+                                      uint32_t* lastk, uint32_t* lastv) {
   using Sort = rocprim::block_radix_sort<uint32_t, 256, ITEMS, uint32_t>;
-  __shared__ typename Sort::storage_type storage;
-  __shared__ uint32_t sk[kCap], sv[kCap];
-  const uint32_t part = blockIdx.x;
-  const uint32_t s = start[part], cnt = end[part] - s;
-  if (cnt < 2) return;  // a lone element is a bucket head: its row entry stays i
-  if (cnt > kCap) {
-    if (threadIdx.x == 0) atomicAdd(overflow, 1u);
-    return;
-  }
+  auto& storage = *reinterpret_cast<typename Sort::storage_type*>(storage_raw);
+  const uint32_t t = threadIdx.x;
   uint32_t key[ITEMS], val[ITEMS];
   #pragma unroll
   for (int j = 0; j < ITEMS; j++) {
-    const uint32_t q = threadIdx.x * ITEMS + j;  // blocked: input order = ascending i
+    const uint32_t q = t * ITEMS + j;  // blocked: input order = ascending i
     if (q < cnt) {
-      const uint64_t v = vals[s + q];
-      key[j] = (uint32_t)(v >> 32);
-      val[j] = (uint32_t)v;
-    } else {  // padding sorts after every element (same top bits, stable)
+      const uint64_t x = v[q];
+      key[j] = (uint32_t)(x >> 32);
+      val[j] = (uint32_t)x;
+    } else {  // padding sorts after every element (largest low bits, stable)
       key[j] = 0xFFFFFFFFu;
       val[j] = 0xFFFFFFFFu;
     }
   }
   // The partition shares the top kPartBits of h: sort the rest, stably.
   Sort().sort(key, val, storage, 0, 32 - kPartBits);
+  lastk[t] = key[ITEMS - 1];
+  lastv[t] = val[ITEMS - 1];
+  __syncthreads();
+  uint32_t pk = t ? lastk[t - 1] : 0u, pv = t ? lastv[t - 1] : 0u;
   #pragma unroll
   for (int j = 0; j < ITEMS; j++) {
-    sk[threadIdx.x * ITEMS + j] = key[j];
-    sv[threadIdx.x * ITEMS + j] = val[j];
+    const uint32_t q = t * ITEMS + j;
+    if (q >= 1 && q < cnt && key[j] == pk) rep[row + val[j]] = pv;
+    pk = key[j];
+    pv = val[j];
   }
-  __syncthreads();
+}
+
+// One block per partition, sorted at the smallest of three capacities
+// (MAXI/4, MAXI/2, MAXI items per thread) that holds it: the sort's cost
+// follows its capacity, and partitions average well under the largest.
+template <int MAXI>
+__global__ __launch_bounds__(256) void k_part_local(const uint64_t* __restrict__ vals,
+                                                    const uint32_t* __restrict__ start,
+                                                    const uint32_t* __restrict__ end, uint32_t n,
+                                                    uint32_t* __restrict__ rep,
+                                                    uint32_t* __restrict__ overflow) {
+  using S1 = rocprim::block_radix_sort<uint32_t, 256, MAXI / 4, uint32_t>;
+  using S2 = rocprim::block_radix_sort<uint32_t, 256, MAXI / 2, uint32_t>;
+  using S3 = rocprim::block_radix_sort<uint32_t, 256, MAXI, uint32_t>;
+  constexpr size_t kStore =
+      std::max(sizeof(typename S1::storage_type),
+               std::max(sizeof(typename S2::storage_type), sizeof(typename S3::storage_type)));
+  __shared__ __align__(16) unsigned char storage[kStore];
+  __shared__ uint32_t lastk[256], lastv[256];
+  const uint32_t part = blockIdx.x;
+  const uint32_t s = start[part], cnt = end[part] - s;
+  if (cnt < 2) return;  // a lone element is a bucket head: its row entry stays i
   const uint64_t row = (uint64_t)(part >> kPartBits) * n;
-  for (uint32_t q = threadIdx.x + 1; q < cnt; q += 256)
-    if (sk[q] == sk[q - 1]) rep[row + sv[q]] = sv[q - 1];
+  if (cnt <= 256u * (MAXI / 4))
+    part_sort_tier<MAXI / 4>(vals + s, cnt, row, rep, storage, lastk, lastv);
+  else if (cnt <= 256u * (MAXI / 2))
+    part_sort_tier<MAXI / 2>(vals + s, cnt, row, rep, storage, lastk, lastv);
+  else if (cnt <= 256u * MAXI)
+    part_sort_tier<MAXI>(vals + s, cnt, row, rep, storage, lastk, lastv);
+  else if (threadIdx.x == 0)
+    atomicAdd(overflow, 1u);
 }
 
 namespace {
