@@ -438,7 +438,7 @@ extern "C" int ngia_db_bucket(ngia_ctx* ctx, ngia_db* db) {
     db->bucket_scratch.ensure(bucket_scratch_bytes(db->n, ctx->bucket_wide));
     NGIA_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
     launch_bucket(db->sig.as<uint32_t>(), db->n, db->rep.as<uint32_t>(), db->bucket_scratch.p,
-                  ctx->stream, ctx->bucket_wide);
+                  ctx->stream, ctx->bucket_wide, db->compact);
     NGIA_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
     NGIA_HIP(hipStreamSynchronize(ctx->stream));
     ctx->timing.bucket_ms = event_ms(ctx->ev[6], ctx->ev[7]);

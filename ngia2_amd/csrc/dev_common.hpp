@@ -64,9 +64,13 @@ __host__ __device__ inline uint32_t lcs_score(uint32_t lcs, uint32_t R, uint32_t
 // the 38-bit-key full sort that large n needs anyway.  Below that, n up to
 // 2.8M takes the partitioned form, which synchronises the stream once (to
 // learn whether a partition overflowed and the full sort must run instead).
+// few_values skips the partitioned form: signatures that take few distinct
+// values per band (8-mers of an alphabet of at most five letters, DNA among
+// them, have at most 5^8 values and their minima crowd near the smallest
+// hashes) make long runs of equal signatures that overflow any partition.
 size_t bucket_scratch_bytes(uint32_t n, bool force_wide = false);
 void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch,
-                   hipStream_t stream, bool force_wide = false);
+                   hipStream_t stream, bool force_wide = false, bool few_values = false);
 
 // Byte-per-residue codes (code of residue j at byte code_off[i] + j, padding
 // to a multiple of 32 is code 0) built from the packed planes.

@@ -185,16 +185,21 @@ __global__ __launch_bounds__(256) void k_bucket_rep_wide(const uint64_t* __restr
 // is replaced, for n <= kPartMaxN, by a partition and a sort in LDS:
 //  1. every (band b, record i) gets h = mix(sig), a bijection of the 32-bit
 //     signature that spreads its bits (minhash values crowd near 0), the
-//     16-bit key (b << 10 | top 10 bits of h) and the value (h << 32 | i);
-//  2. a stable radix sort on the 16-bit key (two passes) groups each band's
-//     elements into 1024 partitions, each still in ascending i;
+//     16-bit key (b << pb | top pb bits of h) and the value (h << 32 | i);
+//  2. a stable radix sort on the key's low pb bits groups each band's
+//     elements into 2^pb partitions, each still in ascending i (the input is
+//     band-major, so the stable sort keeps each (bucket, band) group
+//     contiguous without sorting the band bits);
 //  3. one block per partition sorts it by h in LDS (stable), and every
 //     element equal to its predecessor gets rep[b][i] = that predecessor's i
 //     -- the element before it in the reference's stable (sig, i) order.
-// A partition larger than a block's capacity (heavily repeated signatures)
-// is counted; the host then redoes the whole bucketing with the full sort.
-constexpr uint32_t kPartBits = 10;
-constexpr uint32_t kPartMaxN = 2800000;  // records: partitions average n/1024 <= ~2734
+// pb = 8 (one radix pass, partitions average n/256) up to kPart8MaxN records,
+// else pb = 10 (two passes, partitions average n/1024).  A partition larger
+// than a block's capacity (heavily repeated signatures) is counted; the host
+// then redoes the whole bucketing with the full sort.
+constexpr int kPartBlock = 512;  // threads per partition sort
+constexpr uint32_t kPart8MaxN = 1000000;  // partitions average <= 3907, capacity 8192
+constexpr uint32_t kPartMaxN = 2800000;   // partitions average <= 2735, capacity 4096
 
 __device__ inline uint32_t sig_mix(uint32_t x) {
   x *= 0x9E3779B1u;  // odd multiplier: a bijection mod 2^32
@@ -205,7 +210,7 @@ __device__ inline uint32_t sig_mix(uint32_t x) {
 }
 
 __global__ __launch_bounds__(256) void k_part_keys(const uint32_t* __restrict__ sig, uint32_t n,
-                                                   uint16_t* __restrict__ keys,
+                                                   uint32_t pb, uint16_t* __restrict__ keys,
                                                    uint64_t* __restrict__ vals) {
   // 64 records x 64 bands through LDS, as in k_bucket_keys
   __shared__ uint32_t tile[64][65];
@@ -221,7 +226,7 @@ __global__ __launch_bounds__(256) void k_part_keys(const uint32_t* __restrict__ 
     const uint32_t i = i0 + si;
     if (i < n) {
       const uint32_t h = sig_mix(tile[si][b]);
-      keys[(uint64_t)b * n + i] = (uint16_t)((b << kPartBits) | (h >> (32 - kPartBits)));
+      keys[(uint64_t)b * n + i] = (uint16_t)((b << pb) | (h >> (32 - pb)));
       vals[(uint64_t)b * n + i] = ((uint64_t)h << 32) | i;
     }
   }
@@ -239,15 +244,15 @@ __global__ __launch_bounds__(256) void k_part_bounds(const uint16_t* __restrict_
   }
 }
 
-// One partition sorted in LDS by the low 32 - kPartBits bits of h (stable,
+// One partition sorted in LDS by the low 32 - pb bits of h (stable,
 // so equal values stay in ascending i), ITEMS elements per thread in the
 // blocked arrangement.  The predecessor of a thread's item j is its item
 // j - 1, or for j = 0 the last item of the thread before it.
-template <int ITEMS>
-__device__ inline void part_sort_tier(const uint64_t* __restrict__ v, uint32_t cnt, uint64_t row,
-                                      uint32_t* __restrict__ rep, void* storage_raw,
+template <int BS, int ITEMS>
+__device__ inline void part_sort_tier(const uint64_t* __restrict__ v, uint32_t cnt, uint32_t pb,
+                                      uint64_t row, uint32_t* __restrict__ rep, void* storage_raw,
                                       uint32_t* lastk, uint32_t* lastv) {
-  using Sort = rocprim::block_radix_sort<uint32_t, 256, ITEMS, uint32_t>;
+  using Sort = rocprim::block_radix_sort<uint32_t, BS, ITEMS, uint32_t>;
   auto& storage = *reinterpret_cast<typename Sort::storage_type*>(storage_raw);
   const uint32_t t = threadIdx.x;
   uint32_t key[ITEMS], val[ITEMS];
@@ -263,8 +268,8 @@ __device__ inline void part_sort_tier(const uint64_t* __restrict__ v, uint32_t c
       val[j] = 0xFFFFFFFFu;
     }
   }
-  // The partition shares the top kPartBits of h: sort the rest, stably.
-  Sort().sort(key, val, storage, 0, 32 - kPartBits);
+  // The partition shares the top pb bits of h: sort the rest, stably.
+  Sort().sort(key, val, storage, 0, 32 - pb);
   lastk[t] = key[ITEMS - 1];
   lastv[t] = val[ITEMS - 1];
   __syncthreads();
@@ -278,33 +283,33 @@ __device__ inline void part_sort_tier(const uint64_t* __restrict__ v, uint32_t c
   }
 }
 
-// One block per partition, sorted at the smallest of three capacities
-// (MAXI/4, MAXI/2, MAXI items per thread) that holds it: the sort's cost
-// follows its capacity, and partitions average well under the largest.
-template <int MAXI>
-__global__ __launch_bounds__(256) void k_part_local(const uint64_t* __restrict__ vals,
+// One block of BS threads per partition, sorted at the smallest of three
+// capacities (BS x I1, I2 or I3 items) that holds it: the sort's cost follows
+// its capacity, and most partitions sit near the average, under the largest.
+template <int BS, int I1, int I2, int I3>
+__global__ __launch_bounds__(BS) void k_part_local(const uint64_t* __restrict__ vals,
                                                     const uint32_t* __restrict__ start,
                                                     const uint32_t* __restrict__ end, uint32_t n,
-                                                    uint32_t* __restrict__ rep,
+                                                    uint32_t pb, uint32_t* __restrict__ rep,
                                                     uint32_t* __restrict__ overflow) {
-  using S1 = rocprim::block_radix_sort<uint32_t, 256, MAXI / 4, uint32_t>;
-  using S2 = rocprim::block_radix_sort<uint32_t, 256, MAXI / 2, uint32_t>;
-  using S3 = rocprim::block_radix_sort<uint32_t, 256, MAXI, uint32_t>;
+  using S1 = rocprim::block_radix_sort<uint32_t, BS, I1, uint32_t>;
+  using S2 = rocprim::block_radix_sort<uint32_t, BS, I2, uint32_t>;
+  using S3 = rocprim::block_radix_sort<uint32_t, BS, I3, uint32_t>;
   constexpr size_t kStore =
       std::max(sizeof(typename S1::storage_type),
                std::max(sizeof(typename S2::storage_type), sizeof(typename S3::storage_type)));
   __shared__ __align__(16) unsigned char storage[kStore];
-  __shared__ uint32_t lastk[256], lastv[256];
+  __shared__ uint32_t lastk[BS], lastv[BS];
   const uint32_t part = blockIdx.x;
   const uint32_t s = start[part], cnt = end[part] - s;
   if (cnt < 2) return;  // a lone element is a bucket head: its row entry stays i
-  const uint64_t row = (uint64_t)(part >> kPartBits) * n;
-  if (cnt <= 256u * (MAXI / 4))
-    part_sort_tier<MAXI / 4>(vals + s, cnt, row, rep, storage, lastk, lastv);
-  else if (cnt <= 256u * (MAXI / 2))
-    part_sort_tier<MAXI / 2>(vals + s, cnt, row, rep, storage, lastk, lastv);
-  else if (cnt <= 256u * MAXI)
-    part_sort_tier<MAXI>(vals + s, cnt, row, rep, storage, lastk, lastv);
+  const uint64_t row = (uint64_t)(part >> pb) * n;
+  if (cnt <= (uint32_t)BS * I1)
+    part_sort_tier<BS, I1>(vals + s, cnt, pb, row, rep, storage, lastk, lastv);
+  else if (cnt <= (uint32_t)BS * I2)
+    part_sort_tier<BS, I2>(vals + s, cnt, pb, row, rep, storage, lastk, lastv);
+  else if (cnt <= (uint32_t)BS * I3)
+    part_sort_tier<BS, I3>(vals + s, cnt, pb, row, rep, storage, lastk, lastv);
   else if (threadIdx.x == 0)
     atomicAdd(overflow, 1u);
 }
@@ -318,7 +323,8 @@ struct BucketLayout {
 // overflow count
 struct PartLayout {
   bool use;
-  int items;
+  uint32_t bits;  // partition bits pb
+  int tiers;      // capacity tiers: 0 = 4096/5120/8192, 1 = 512/1024/2048, 2 = 1024/2048/4096
   size_t keys_in, keys_out, vals_in, vals_out, start, end, overflow, temp, temp_bytes, total;
 };
 PartLayout part_layout(uint32_t n, bool force_wide) {
@@ -326,13 +332,14 @@ PartLayout part_layout(uint32_t n, bool force_wide) {
   L.use = !force_wide && n <= kPartMaxN;
   if (!L.use) return L;
   const uint64_t m = (uint64_t)n * 64;
-  // capacity at least twice the average partition, n / 1024
-  L.items = n <= 1400000u ? 8 : 16;
+  // The largest capacity is 1.5 to 2 times the average partition, n >> pb.
+  L.bits = n <= kPart8MaxN ? 8 : 10;
+  L.tiers = n <= kPart8MaxN ? 0 : n <= 1400000u ? 1 : 2;
   size_t temp_bytes = 0;
   (void)rocprim::radix_sort_pairs(nullptr, temp_bytes, (uint16_t*)nullptr, (uint16_t*)nullptr,
-                                  (uint64_t*)nullptr, (uint64_t*)nullptr, (size_t)m, 0, 16);
+                                  (uint64_t*)nullptr, (uint64_t*)nullptr, (size_t)m, 0, L.bits);
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  const size_t parts = (size_t)64 << kPartBits;
+  const size_t parts = (size_t)64 << L.bits;
   L.keys_in = 0;
   L.keys_out = L.keys_in + al(2 * m);
   L.vals_in = L.keys_out + al(2 * m);
@@ -379,7 +386,7 @@ namespace {
 bool bucket_partitioned(const PartLayout& L, const uint32_t* sig, uint32_t n, uint32_t* rep,
                         char* s, hipStream_t stream) {
   const uint64_t m = (uint64_t)n * 64;
-  const uint32_t parts = 64u << kPartBits;
+  const uint32_t pb = L.bits, parts = 64u << pb;
   uint16_t* keys_in = (uint16_t*)(s + L.keys_in);
   uint16_t* keys_out = (uint16_t*)(s + L.keys_out);
   uint64_t* vals_in = (uint64_t*)(s + L.vals_in);
@@ -387,23 +394,28 @@ bool bucket_partitioned(const PartLayout& L, const uint32_t* sig, uint32_t n, ui
   uint32_t* start = (uint32_t*)(s + L.start);
   uint32_t* end = (uint32_t*)(s + L.end);
   uint32_t* overflow = (uint32_t*)(s + L.overflow);
-  hipLaunchKernelGGL(k_part_keys, dim3((n + 63) / 64), dim3(256), 0, stream, sig, n, keys_in,
+  hipLaunchKernelGGL(k_part_keys, dim3((n + 63) / 64), dim3(256), 0, stream, sig, n, pb, keys_in,
                      vals_in);
   NGIA_HIP(hipGetLastError());
   size_t tb = L.temp_bytes;
   NGIA_HIP(rocprim::radix_sort_pairs(s + L.temp, tb, keys_in, keys_out, vals_in, vals_out, (size_t)m,
-                                     0, 16, stream));
+                                     0, pb, stream));
   fill_u32(start, 0, parts, stream);
   fill_u32(end, 0, parts, stream);
   fill_u32(overflow, 0, 1, stream);
   hipLaunchKernelGGL(k_part_bounds, dim3(std::min<uint64_t>((m + 255) / 256, 16384)), dim3(256), 0,
                      stream, keys_out, m, start, end);
-  if (L.items == 8)
-    hipLaunchKernelGGL(k_part_local<8>, dim3(parts), dim3(256), 0, stream, vals_out, start, end, n,
-                       rep, overflow);
-  else
-    hipLaunchKernelGGL(k_part_local<16>, dim3(parts), dim3(256), 0, stream, vals_out, start, end, n,
-                       rep, overflow);
+  // capacity tiers around the average partition, n >> pb
+  constexpr int B = kPartBlock;
+  if (L.tiers == 0)
+    hipLaunchKernelGGL((k_part_local<B, 4096 / B, 5120 / B, 8192 / B>), dim3(parts), dim3(B), 0,
+                       stream, vals_out, start, end, n, pb, rep, overflow);
+  else if (L.tiers == 1)
+    hipLaunchKernelGGL((k_part_local<B, 512 / B, 1024 / B, 2048 / B>), dim3(parts), dim3(B), 0,
+                       stream, vals_out, start, end, n, pb, rep, overflow);
+  else  // 256 threads measured faster here than 512 (2M records: 3.9 against 4.2 ms)
+    hipLaunchKernelGGL((k_part_local<256, 4, 8, 16>), dim3(parts), dim3(256), 0, stream, vals_out,
+                       start, end, n, pb, rep, overflow);
   NGIA_HIP(hipGetLastError());
   uint32_t over = 0;
   NGIA_HIP(hipMemcpyAsync(&over, overflow, 4, hipMemcpyDeviceToHost, stream));
@@ -413,7 +425,7 @@ bool bucket_partitioned(const PartLayout& L, const uint32_t* sig, uint32_t n, ui
 }  // namespace
 
 void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch,
-                   hipStream_t stream, bool force_wide) {
+                   hipStream_t stream, bool force_wide, bool few_values) {
   if (n == 0) return;
   const BucketLayout L = bucket_layout(n, force_wide);
   char* s = (char*)scratch;
@@ -424,7 +436,7 @@ void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch
   hipLaunchKernelGGL(k_rep_identity, dim3(grid_for(n, 256), 64), dim3(256), 0, stream, n, rep);
   NGIA_HIP(hipGetLastError());
   const PartLayout P = part_layout(n, force_wide);
-  if (P.use) {
+  if (P.use && !few_values) {
     if (bucket_partitioned(P, sig, n, rep, s, stream)) return;
     // a partition overflowed: start again from the identity rows, full sort
     hipLaunchKernelGGL(k_rep_identity, dim3(grid_for(n, 256), 64), dim3(256), 0, stream, n, rep);

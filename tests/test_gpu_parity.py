@@ -104,17 +104,20 @@ def test_bucket_collisions(ctx):
 
 def test_bucket_partition_collisions(ctx):
     """Many equal signatures inside the partitions' capacity (the sort in LDS
-    orders up to ~40 copies of each value per band), and a band where every
-    record has the same signature, so that one partition overflows and the
-    bucketing is redone with the full sort."""
+    orders up to ~40 copies of each value per band); a band where all 5,000
+    records share one signature (one partition of 5,000 sorted in LDS, the
+    largest capacity tier); and 9,000 such records, beyond the largest
+    capacity (8,192), so that the partition overflows and the bucketing is
+    redone with the full sort."""
     rng = np.random.default_rng(8)
     sig = rng.integers(0, 50, size=(2000, 64)).astype(np.uint32) * np.uint32(0x01000193)
     say("bucket partition collisions", sig.shape)
     np.testing.assert_array_equal(ctx.bucket(sig), orc.bucket(sig))
-    over = rng.integers(0, 1 << 32, size=(5000, 64), dtype=np.uint64).astype(np.uint32)
-    over[:, 3] = 12345
-    say("bucket partition overflow", over.shape)
-    np.testing.assert_array_equal(ctx.bucket(over), orc.bucket(over))
+    for n in (5000, 9000):
+        big = rng.integers(0, 1 << 32, size=(n, 64), dtype=np.uint64).astype(np.uint32)
+        big[:, 3] = 12345
+        say("bucket one band of equal signatures", big.shape)
+        np.testing.assert_array_equal(ctx.bucket(big), orc.bucket(big))
 
 
 def family_sigs(rng, n, fam=20, keep=0.7):
@@ -126,10 +129,11 @@ def family_sigs(rng, n, fam=20, keep=0.7):
     return np.where(rng.random((n, 64)) < keep, sig, noise)
 
 
-@pytest.mark.parametrize("n", [1000000, 1500000])
+@pytest.mark.parametrize("n", [1000000, 1000001, 1500000])
 def test_bucket_partitioned_large(ctx, n):
-    """At a million records and more (both block capacities of the partitioned
-    form) the rows equal those of the 38-bit full sort."""
+    """At a million records (8-bit partitions, one radix pass) and beyond
+    (10-bit partitions, both block capacities) the rows equal those of the
+    38-bit full sort."""
     sig = family_sigs(np.random.default_rng(n), n)
     say("bucket partitioned vs full sort", n)
     got = ctx.bucket(sig)
