@@ -128,6 +128,29 @@ int ngia_build_graph(ngia_ctx* ctx, const uint32_t* packed, uint64_t n_words,
                      uint32_t flags, ngia_edge** edges, uint64_t* n_edges,
                      ngia_graph_stats* stats);
 
+/* (3') edge emission for a database larger than the device holds.  No kernel
+ * counterpart: the reference keeps the packed area in managed memory and
+ * lets the driver page it (README.md:8, buildGraph.hpp:143, 164).  The
+ * records are cut into blocks of consecutive indices of at most block_words
+ * packed words each (a longer record is a block by itself); every candidate
+ * pair (rep r, target t) is aligned in the chunk of r's block and t's block,
+ * which is all that is uploaded for it, in the batched form.  The edge list
+ * equals ngia_build_graph's.  block_words = 0 sizes blocks from the free
+ * device memory (two blocks within half of it), which is one block when the
+ * whole database fits.  *n_chunks (may be NULL) receives the chunks run. */
+int ngia_build_graph_chunked(ngia_ctx* ctx, const uint32_t* packed, uint64_t n_words,
+                             const uint64_t* word_off, const uint32_t* lengths, uint32_t n,
+                             const uint32_t* rep_rows, uint32_t bands, uint32_t identity,
+                             uint64_t block_words, ngia_edge** edges, uint64_t* n_edges,
+                             ngia_graph_stats* stats, uint32_t* n_chunks);
+/* The block cut of ngia_build_graph_chunked: records in index order, a new
+ * block when the next record would take the block past block_words packed
+ * words (1 + 4 ceil(L/32) per record) or past block_records records
+ * (0 = no limit).  bounds holds n + 1 entries; block k is
+ * [bounds[k], bounds[k+1]).  Host only. */
+int ngia_plan_blocks(const uint32_t* lengths, uint32_t n, uint64_t block_words,
+                     uint32_t block_records, uint32_t* bounds, uint32_t* n_blocks);
+
 /* ------------------------------------------- device-resident database */
 /* Upload the packed area, word offsets and lengths once (BuildGraph::readData,
  * buildGraph.hpp:125-183, with explicit device buffers instead of managed

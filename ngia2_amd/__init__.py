@@ -169,6 +169,27 @@ class Context:
             _lib.lib().ngia_free(ep)
         return arr, st.as_dict()
 
+    def build_graph_chunked(self, packed, word_off, lengths, rep_rows, bands, identity,
+                            block_words=0):
+        """Edges of the whole database, aligned chunk by chunk (pairs of blocks
+        of at most block_words packed words; 0 sizes them from free device
+        memory).  Returns (edges, stats, chunks)."""
+        packed, word_off, lengths = _u32(packed), _u64(word_off), _u32(lengths)
+        rep_rows = _u32(rep_rows)
+        ep = C.POINTER(_lib.Edge)()
+        m = C.c_uint64()
+        st = _lib.GraphStats()
+        nc = C.c_uint32()
+        _check(_lib.lib().ngia_build_graph_chunked(
+            self.h, _ptr(packed), len(packed), _ptr(word_off), _ptr(lengths), len(lengths),
+            _ptr(rep_rows), bands, identity, block_words, C.byref(ep), C.byref(m), C.byref(st),
+            C.byref(nc)), self.h)
+        try:
+            arr = edges_to_array(ep, m.value)
+        finally:
+            _lib.lib().ngia_free(ep)
+        return arr, st.as_dict(), nc.value
+
     def makedb(self, fasta, packed):
         _check(_lib.lib().ngia_makedb(self.h, fasta.encode(), packed.encode()), self.h)
 
@@ -401,6 +422,16 @@ def read_db(path, bands=BANDS):
     raw = _lib.PackedDB()
     _check(_lib.lib().ngia_read_db(path.encode(), bands, C.byref(raw)))
     return PackedDB._from_raw(raw, bands)
+
+
+def plan_blocks(lengths, block_words, block_records=0):
+    """Block cut of build_graph_chunked (host only): the list of block bounds."""
+    lengths = _u32(lengths)
+    bounds = np.zeros(len(lengths) + 1, dtype=np.uint32)
+    nb = C.c_uint32()
+    _check(_lib.lib().ngia_plan_blocks(_ptr(lengths), len(lengths), block_words, block_records,
+                                       _ptr(bounds), C.byref(nb)))
+    return bounds[:nb.value + 1].copy()
 
 
 def gen_config(config, scale=1.0, seed=1):

@@ -169,6 +169,10 @@ PROTOTYPES = {
     "ngia_build_graph": (C.c_int, [P, P, U64, P, P, U32, P, U32, U32, U32,
                                    C.POINTER(C.POINTER(Edge)), C.POINTER(U64),
                                    C.POINTER(GraphStats)]),
+    "ngia_build_graph_chunked": (C.c_int, [P, P, U64, P, P, U32, P, U32, U32, U64,
+                                           C.POINTER(C.POINTER(Edge)), C.POINTER(U64),
+                                           C.POINTER(GraphStats), C.POINTER(U32)]),
+    "ngia_plan_blocks": (C.c_int, [P, U32, U64, U32, P, C.POINTER(U32)]),
     "ngia_db_create": (C.c_int, [P, P, U64, P, P, U32, C.POINTER(P)]),
     "ngia_db_destroy": (None, [P]),
     "ngia_db_sketch": (C.c_int, [P, P]),

@@ -233,14 +233,28 @@ __global__ __launch_bounds__(256) void k_part_keys(const uint32_t* __restrict__ 
 }
 
 // start[k], end[k]: the sorted range of partition k (left 0, 0 when empty).
+// Eight keys per thread from one 16-byte load (m is a multiple of 64), the
+// neighbours across the group boundary from the adjacent groups.
 __global__ __launch_bounds__(256) void k_part_bounds(const uint16_t* __restrict__ keys, uint64_t m,
                                                      uint32_t* __restrict__ start,
                                                      uint32_t* __restrict__ end) {
-  for (uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x; p < m;
-       p += (uint64_t)gridDim.x * 256) {
-    const uint32_t k = keys[p];
-    if (p == 0 || keys[p - 1] != k) start[k] = (uint32_t)p;
-    if (p + 1 == m || keys[p + 1] != k) end[k] = (uint32_t)(p + 1);
+  const uint64_t groups = m / 8;
+  for (uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x; g < groups;
+       g += (uint64_t)gridDim.x * 256) {
+    const uint64_t p0 = 8 * g;
+    const uint4 w = *(const uint4*)(keys + p0);
+    uint32_t k[8] = {w.x & 0xFFFFu, w.x >> 16, w.y & 0xFFFFu, w.y >> 16,
+                     w.z & 0xFFFFu, w.z >> 16, w.w & 0xFFFFu, w.w >> 16};
+    // keys are 16-bit: 0x10000 differs from every key
+    uint32_t prev = p0 ? keys[p0 - 1] : 0x10000u;
+    const uint32_t after = p0 + 8 < m ? keys[p0 + 8] : 0x10000u;
+    #pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const uint32_t next = j < 7 ? k[j + 1] : after;
+      if (prev != k[j]) start[k[j]] = (uint32_t)(p0 + j);
+      if (next != k[j]) end[k[j]] = (uint32_t)(p0 + j + 1);
+      prev = k[j];
+    }
   }
 }
 
@@ -403,8 +417,8 @@ bool bucket_partitioned(const PartLayout& L, const uint32_t* sig, uint32_t n, ui
   fill_u32(start, 0, parts, stream);
   fill_u32(end, 0, parts, stream);
   fill_u32(overflow, 0, 1, stream);
-  hipLaunchKernelGGL(k_part_bounds, dim3(std::min<uint64_t>((m + 255) / 256, 16384)), dim3(256), 0,
-                     stream, keys_out, m, start, end);
+  hipLaunchKernelGGL(k_part_bounds, dim3(std::min<uint64_t>((m / 8 + 255) / 256, 16384)), dim3(256),
+                     0, stream, keys_out, m, start, end);
   // capacity tiers around the average partition, n >> pb
   constexpr int B = kPartBlock;
   if (L.tiers == 0)

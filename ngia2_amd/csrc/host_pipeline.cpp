@@ -210,18 +210,28 @@ extern "C" int ngia_cluster(ngia_ctx* ctx, const char* packed_path, const char* 
   if (rc != NGIA_OK) return rc;
   std::printf("reads count:\t%u\nlongest:\t%u\nshortest:\t%u\n", pdb.n, pdb.lengths[0],
               pdb.lengths[pdb.n - 1]);
-  ngia_db* db = nullptr;
   std::vector<ngia_edge> edges;
-  rc = ngia_db_create(ctx, pdb.packed, pdb.n_words, pdb.word_off, pdb.lengths, pdb.n, &db);
-  if (rc == NGIA_OK) rc = ngia_db_set_reps(ctx, db, pdb.rep_rows, bands);
-  if (rc == NGIA_OK) rc = ngia_db_graph(ctx, db, identity, bands, flags, stats);
   uint64_t m = 0;
-  if (rc == NGIA_OK) rc = ngia_db_get_edges(ctx, db, nullptr, 0, &m);
-  if (rc == NGIA_OK) {
-    edges.resize(m);
-    rc = ngia_db_get_edges(ctx, db, edges.data(), m, &m);
+  if (flags == NGIA_GRAPH_BATCHED) {
+    // The whole database on the device when it fits; else blocks of it, the
+    // part the reference leaves to managed-memory paging (README.md:8).
+    ngia_edge* e = nullptr;
+    rc = ngia_build_graph_chunked(ctx, pdb.packed, pdb.n_words, pdb.word_off, pdb.lengths, pdb.n,
+                                  pdb.rep_rows, bands, identity, 0, &e, &m, stats, nullptr);
+    if (rc == NGIA_OK) edges.assign(e, e + m);
+    ngia_free(e);
+  } else {
+    ngia_db* db = nullptr;
+    rc = ngia_db_create(ctx, pdb.packed, pdb.n_words, pdb.word_off, pdb.lengths, pdb.n, &db);
+    if (rc == NGIA_OK) rc = ngia_db_set_reps(ctx, db, pdb.rep_rows, bands);
+    if (rc == NGIA_OK) rc = ngia_db_graph(ctx, db, identity, bands, flags, stats);
+    if (rc == NGIA_OK) rc = ngia_db_get_edges(ctx, db, nullptr, 0, &m);
+    if (rc == NGIA_OK) {
+      edges.resize(m);
+      rc = ngia_db_get_edges(ctx, db, edges.data(), m, &m);
+    }
+    ngia_db_destroy(db);
   }
-  ngia_db_destroy(db);
   const uint32_t n = pdb.n;
   ngia_release_db(&pdb);
   if (rc != NGIA_OK) return rc;

@@ -608,3 +608,49 @@ def test_sketch_past_2gib_of_codes(ctx):
     for idx in (head, tail):
         want = np.stack([orc.sketch_record(d.packed[int(d.word_off[i]):]) for i in idx])
         np.testing.assert_array_equal(got[idx], want)
+
+
+@pytest.mark.parametrize("identity", [50, 90])
+def test_chunked_graph_c1(ctx, c1db, identity):
+    """The out-of-HBM form: blocks of about a fifth of the packed area, every
+    (rep block, target block) chunk uploaded and aligned alone; the merged
+    edges and the candidate count equal the oracle's."""
+    _, _, d = c1db
+    words = int(d["word_off"][-1]) + 1
+    say("chunked graph c1 identity", identity, "block words", words // 5)
+    got, st, chunks = ctx.build_graph_chunked(d["packed"], d["word_off"], d["lengths"], d["rep"],
+                                              64, identity, block_words=words // 5)
+    want, tot = orc.build_graph(d["packed"], d["word_off"], d["lengths"], d["rep"], 64, identity)
+    assert chunks > 5
+    assert st["total_pairs"] == tot
+    np.testing.assert_array_equal(got, want)
+
+
+def test_chunked_graph_single_records(ctx, c1db):
+    """Every record a block of its own (block_words = 1): one chunk per
+    (representative, target) pair; edges equal the oracle's."""
+    _, _, d = c1db
+    n = 400
+    rep = np.ascontiguousarray(d["rep"][:, :n])
+    say("chunked graph, one record per block, records", n)
+    got, st, chunks = ctx.build_graph_chunked(d["packed"], d["word_off"][:n], d["lengths"][:n], rep,
+                                              64, 70, block_words=1)
+    want, tot = orc.build_graph(d["packed"], d["word_off"][:n], d["lengths"][:n], rep, 64, 70)
+    assert st["total_pairs"] == tot and chunks == st["distinct_pairs"]
+    np.testing.assert_array_equal(got, want)
+
+
+def test_chunked_graph_long_and_whole(ctx, longdb):
+    """Long records, each longer than the block limit (blocks of one record),
+    and the automatic size (the whole set fits: one chunk, the same edges)."""
+    _, _, d = longdb
+    say("chunked graph long records")
+    want, tot = orc.build_graph(d["packed"], d["word_off"], d["lengths"], d["rep"], 64, 90)
+    got, st, chunks = ctx.build_graph_chunked(d["packed"], d["word_off"], d["lengths"], d["rep"],
+                                              64, 90, block_words=64)
+    assert chunks > 1 and st["total_pairs"] == tot
+    np.testing.assert_array_equal(got, want)
+    got, st, chunks = ctx.build_graph_chunked(d["packed"], d["word_off"], d["lengths"], d["rep"],
+                                              64, 90)
+    assert chunks == 1
+    np.testing.assert_array_equal(got, want)

@@ -187,3 +187,29 @@ def test_bench_refuses_several_gpus():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_plan_blocks():
+    """The block cut of the out-of-HBM graph: consecutive records, a new block
+    when the next record would pass the word or record limit, an oversized
+    record alone, every record in exactly one block."""
+    rng = np.random.default_rng(5)
+    lengths = rng.integers(0, 3000, size=500).astype(np.uint32)
+    words = 1 + 4 * ((lengths.astype(np.int64) + 31) // 32)
+    for block_words, block_records in ((1, 0), (400, 0), (5000, 0), (5000, 7), (10 ** 9, 0),
+                                       (10 ** 9, 100)):
+        b = ng.plan_blocks(lengths, block_words, block_records)
+        assert b[0] == 0 and b[-1] == len(lengths) and np.all(np.diff(b.astype(np.int64)) > 0)
+        for k in range(len(b) - 1):
+            w = words[b[k]:b[k + 1]]
+            # within the limits, or a single record that alone exceeds them
+            assert len(w) == 1 or w.sum() <= block_words
+            assert block_records == 0 or len(w) <= block_records
+            if k + 1 < len(b) - 1:  # greedy: the next record did not fit
+                nxt = words[b[k + 1]]
+                assert w.sum() + nxt > block_words or (block_records and len(w) == block_records)
+    assert len(ng.plan_blocks(lengths, 1)) == len(lengths) + 1
+    assert len(ng.plan_blocks(lengths, 10 ** 9)) == 2
+    assert list(ng.plan_blocks(np.zeros(0, np.uint32), 10)) == [0]
+    with pytest.raises(ng.NgiaError):
+        ng.plan_blocks(lengths, 0)
