@@ -121,9 +121,15 @@ void graph_plain(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
   }
 }
 
-// Enqueue one batched round (capturable: fixed grids, device-side counts).
-void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
-                     bool time_align) {
+// A batched round runs in two captured parts.  The first finds the distinct
+// candidate pairs and bins them; the host then reads the bin counts, and the
+// second launches only the alignment kernels whose bins hold pairs.  An empty
+// bin's kernel would still have to place all its persistent blocks, and a
+// wide window kernel (55-68 KiB of LDS per block) waits for CUs the busy bins
+// hold, stalling every kernel queued behind it on its stream.
+
+// Part 1: candidates and bins (capturable: fixed grids, device-side counts).
+void enqueue_bins(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands) {
   hipStream_t s = ctx->stream;
   const uint32_t n = db->n;
   const float thr = (float)identity / 100.0f;
@@ -140,25 +146,38 @@ void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ban
   bin_pairs(db->pairs.as<uint2>(), npairs, len, thr, db->bin_of.as<uint32_t>(), dc->bin_cnt,
             dc->bin_off, dc->cursor, db->list.as<uint32_t>(), dc->bin_tiles, db->score.as<uint32_t>(),
             ctx->wave_kmax, dc->remap, ctx->route, grid, s);
+}
+
+// Part 2: the alignment kernels of the bins in `mask` (bit b: bin b holds
+// pairs), then the passing pairs as raw edges (capturable).
+void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
+                   bool time_align) {
+  hipStream_t s = ctx->stream;
+  const uint32_t n = db->n;
+  const float thr = (float)identity / 100.0f;
+  Counters* dc = db->counters.as<Counters>();
+  const uint64_t* npairs = db->off.as<uint64_t>() + n;
+  const uint32_t grid = persistent_blocks(ctx, 0) * 4;
   const AlignArgs aa = align_args(db, thr);
+  auto holds = [&](int bin) { return ((mask >> bin) & 1u) != 0; };
   // The bins run concurrently, forked onto the side streams and joined back.
   // Every wave bin below K = 16 is latency bound (one pair per wavefront, the
   // bin lasts as long as its longest pair), so each gets a side stream of its
   // own and starts at the fork: K = 8 on side stream 0, K = 4 on 1, K = 2 on
-  // 2, K = 1 on 3.  The window bins (throughput bound) follow round-robin on
-  // side streams 0, 3 and 2, widest first.  K = 32 and 16 (128 and 64 KiB of
-  // LDS per block) run before the fork: launched beside the others, their
-  // blocks wait for a CU with that much free LDS, which the other bins'
-  // persistent blocks hold until they finish, and that wait would stall
-  // their stream even when the bin is empty.
+  // 2, K = 1 on 3.  The window bins (throughput bound) that hold pairs follow
+  // round-robin on side streams 0, 3 and 2, widest first.  K = 32 and 16 (128
+  // and 64 KiB of LDS per block), when they hold pairs, run before the fork:
+  // launched beside the others, their blocks would wait for a CU with that
+  // much free LDS, which the other bins' persistent blocks hold until they
+  // finish.
   if (time_align) NGIA_HIP(hipEventRecord(ctx->ev[2], s));
   const bool fork = !ctx->serial_bins;
   auto mark = [&](int bin, int end, hipStream_t q) {
     if (time_align) NGIA_HIP(hipEventRecord(ctx->bin_ev[2 * bin + end], q));
   };
   auto wave = [&](int k, hipStream_t q) {
-    if ((1u << k) > ctx->wave_kmax) return;
     const int bin = kBinWave0 + k;
+    if ((1u << k) > ctx->wave_kmax || !holds(bin)) return;
     mark(bin, 0, q);
     launch_align_wave(1 << k, aa, db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[bin],
                       &dc->bin_cnt[bin], &dc->work[bin], db->score.as<uint32_t>(),
@@ -175,6 +194,7 @@ void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ban
   static constexpr int kWinStreams[3] = {0, 3, 2};
   int slot = 0;
   for (int W = kWinMax; W >= 2; W--) {
+    if (!holds(W)) continue;
     hipStream_t q = fork ? ctx->side[kWinStreams[slot++ % 3]] : s;
     mark(W, 0, q);
     launch_align_win(W, aa, db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[W],
@@ -188,18 +208,40 @@ void enqueue_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ban
   }
   if (time_align) NGIA_HIP(hipEventRecord(ctx->ev[3], s));
   // windows beyond 64*32 words: the plain kernel over job triplets
-  plain_bin_jobs(db->pairs.as<uint2>(), db->list.as<uint32_t>(), dc->bin_off, dc->bin_cnt,
-                 db->plain_jobs.as<uint32_t>(), s);
-  const uint32_t nw_max = (db->max_len + 31) / 32;
-  mark(kBinPlain, 0, s);
-  launch_align_plain(aa, db->plain_jobs.as<uint32_t>(), 0xFFFFFFFFu, nw_max,
-                     db->plain_scratch.as<uint32_t>(), db->plain_scratch.bytes / 4, s,
-                     &dc->bin_cnt[kBinPlain]);
-  mark(kBinPlain, 1, s);
-  plain_bin_scores(db->list.as<uint32_t>(), dc->bin_off, dc->bin_cnt, db->plain_jobs.as<uint32_t>(),
-                   db->score.as<uint32_t>(), s);
+  if (holds(kBinPlain)) {
+    plain_bin_jobs(db->pairs.as<uint2>(), db->list.as<uint32_t>(), dc->bin_off, dc->bin_cnt,
+                   db->plain_jobs.as<uint32_t>(), s);
+    const uint32_t nw_max = (db->max_len + 31) / 32;
+    mark(kBinPlain, 0, s);
+    launch_align_plain(aa, db->plain_jobs.as<uint32_t>(), 0xFFFFFFFFu, nw_max,
+                       db->plain_scratch.as<uint32_t>(), db->plain_scratch.bytes / 4, s,
+                       &dc->bin_cnt[kBinPlain]);
+    mark(kBinPlain, 1, s);
+    plain_bin_scores(db->list.as<uint32_t>(), dc->bin_off, dc->bin_cnt,
+                     db->plain_jobs.as<uint32_t>(), db->score.as<uint32_t>(), s);
+  }
   pairs_to_edges(db->pairs.as<uint2>(), npairs, db->score.as<uint32_t>(), db->ekeys.as<uint64_t>(),
                  db->escores.as<uint32_t>(), &dc->ecount, grid, s);
+}
+
+// Capture what fn enqueues on stream s into an instantiated graph.
+template <class F>
+hipGraphExec_t capture(hipStream_t s, F fn) {
+  hipGraph_t g = nullptr;
+  NGIA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    fn();
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  NGIA_HIP(hipStreamEndCapture(s, &g));
+  hipGraphExec_t ge = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  NGIA_HIP(e);
+  return ge;
 }
 
 // Workspace of one batched round over db (sized for every target).
@@ -241,33 +283,38 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   for (uint32_t w : ctx->route.route_below) key.push_back(w);
   const bool reuse = db->gexec && db->g_identity == identity && db->g_bands == bands &&
                      db->g_key == key;
-  NGIA_HIP(hipEventRecord(ctx->ev[0], s));
-  if (eager) {
-    // Same kernels without the graph, with events around the alignment phase.
-    enqueue_batched(ctx, db, identity, bands, /*time_align=*/true);
-  } else if (!reuse) {
-    if (db->gexec) {
-      (void)hipGraphExecDestroy(db->gexec);
-      db->gexec = nullptr;
-    }
-    hipGraph_t g = nullptr;
-    NGIA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    try {
-      enqueue_batched(ctx, db, identity, bands, false);
-    } catch (...) {
-      (void)hipStreamEndCapture(s, &g);
-      if (g) (void)hipGraphDestroy(g);
-      throw;
-    }
-    NGIA_HIP(hipStreamEndCapture(s, &g));
-    hipError_t e = hipGraphInstantiate(&db->gexec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    NGIA_HIP(e);
+  if (!eager && !reuse) {
+    drop_graphs(db);
+    db->gexec = capture(s, [&] { enqueue_bins(ctx, db, identity, bands); });
     db->g_identity = identity;
     db->g_bands = bands;
     db->g_key = key;
   }
-  if (!eager) NGIA_HIP(hipGraphLaunch(db->gexec, s));
+  NGIA_HIP(hipEventRecord(ctx->ev[0], s));
+  if (eager)
+    enqueue_bins(ctx, db, identity, bands);
+  else
+    NGIA_HIP(hipGraphLaunch(db->gexec, s));
+  // the bins that hold pairs pick the second part
+  uint32_t cnt[kBins];
+  d2h(cnt, db->counters.as<Counters>()->bin_cnt, sizeof(cnt), s);
+  NGIA_HIP(hipStreamSynchronize(s));
+  uint32_t mask = 0;
+  for (int b = 0; b < kBins; b++)
+    if (cnt[b]) mask |= 1u << b;
+  if (eager) {
+    // Same kernels without the graph, with events around the alignment phase.
+    enqueue_align(ctx, db, identity, mask, /*time_align=*/true);
+  } else {
+    hipGraphExec_t g2 = nullptr;
+    for (const auto& e : db->gexec2)
+      if (e.first == mask) g2 = e.second;
+    if (!g2) {
+      g2 = capture(s, [&] { enqueue_align(ctx, db, identity, mask, false); });
+      db->gexec2.emplace_back(mask, g2);
+    }
+    NGIA_HIP(hipGraphLaunch(g2, s));
+  }
   NGIA_HIP(hipEventRecord(ctx->ev[1], s));
   Counters hc;
   d2h(&hc, db->counters.p, sizeof(Counters), s);
@@ -284,9 +331,12 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   ctx->timing.align_ms = eager ? event_ms(ctx->ev[2], ctx->ev[3]) : 0.0;
   db->last = hc;
   db->have_bins = true;
+  uint32_t launches = 0;
   for (int b = 0; b < kBins; b++) {
-    const bool launched = b == kBinPlain || (b >= 2 && b <= kWinMax) ||
-                          (b >= kBinWave0 && b < kBinPlain && (1u << (b - kBinWave0)) <= ctx->wave_kmax);
+    const bool aligner = b == kBinPlain || (b >= 2 && b <= kWinMax) ||
+                         (b >= kBinWave0 && b < kBinPlain && (1u << (b - kBinWave0)) <= ctx->wave_kmax);
+    const bool launched = aligner && ((mask >> b) & 1u);
+    launches += launched;
     db->bin_ms[b] = eager && launched ? event_ms(ctx->bin_ev[2 * b], ctx->bin_ev[2 * b + 1]) : 0.f;
   }
   uint64_t tiles = 0;
@@ -299,8 +349,8 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
     st->ms_candidates = 0;
     st->ms_align = event_ms(ctx->ev[0], ctx->ev[1]);
     st->ms_edges = event_ms(ctx->ev[4], ctx->ev[5]);
-    st->align_launches = 6 + (kWinMax - 1) + 1;
-    st->host_syncs = 2;
+    st->align_launches = launches;
+    st->host_syncs = 3;
   }
 }
 
@@ -459,10 +509,7 @@ extern "C" int ngia_db_set_reps(ngia_ctx* ctx, ngia_db* db, const uint32_t* rep_
     h2d(db->rep.p, rep_rows, 4 * m, ctx->stream);
     NGIA_HIP(hipStreamSynchronize(ctx->stream));
     db->rep_bands = bands;
-    if (db->gexec) {
-      (void)hipGraphExecDestroy(db->gexec);
-      db->gexec = nullptr;
-    }
+    drop_graphs(db);
     return NGIA_OK;
   });
 }

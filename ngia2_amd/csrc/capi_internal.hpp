@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "dev_common.hpp"
@@ -123,17 +124,28 @@ struct ngia_db {
   uint64_t n_edges = 0, n_pairs = 0;
   // the captured batched round; g_key holds the buffer addresses and the
   // routing it was captured with
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec = nullptr;  // candidates and bins
   uint32_t g_identity = 0, g_bands = 0;
   std::vector<uintptr_t> g_key;
+  // the alignment part, one capture per set of bins that hold pairs
+  std::vector<std::pair<uint32_t, hipGraphExec_t>> gexec2;
   // per-bin figures of the last batched graph
   bool have_bins = false;
   ngia::Counters last{};
   float bin_ms[ngia::kBins] = {};
   ~ngia_db() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
+    for (auto& e : gexec2) (void)hipGraphExecDestroy(e.second);
   }
 };
+
+// Drop the captured rounds of db (its buffers or routing changed).
+inline void drop_graphs(ngia_db* db) {
+  if (db->gexec) (void)hipGraphExecDestroy(db->gexec);
+  db->gexec = nullptr;
+  for (auto& e : db->gexec2) (void)hipGraphExecDestroy(e.second);
+  db->gexec2.clear();
+}
 
 // Called by ngia_close for each live serving handle (serve.hip).
 void serve_detach(ngia_serve* sv);
@@ -260,10 +272,7 @@ inline void db_upload(ngia_ctx* ctx, ngia_db* db, const uint32_t* packed, uint64
   NGIA_HIP(hipStreamSynchronize(s));
   db->have_sig = false;
   db->rep_bands = 0;
-  if (db->gexec) {
-    (void)hipGraphExecDestroy(db->gexec);
-    db->gexec = nullptr;
-  }
+  drop_graphs(db);
 }
 
 inline AlignArgs align_args(const ngia_db* db, float thr) {
