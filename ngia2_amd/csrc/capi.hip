@@ -310,6 +310,10 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
     for (const auto& e : db->gexec2)
       if (e.first == mask) g2 = e.second;
     if (!g2) {
+      if (db->gexec2.size() >= 16) {  // bounded: start afresh when the bins keep changing
+        for (auto& e : db->gexec2) (void)hipGraphExecDestroy(e.second);
+        db->gexec2.clear();
+      }
       g2 = capture(s, [&] { enqueue_align(ctx, db, identity, mask, false); });
       db->gexec2.emplace_back(mask, g2);
     }
