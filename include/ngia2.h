@@ -49,7 +49,7 @@ typedef struct {
 
 /* Graph-build modes (flags of ngia_db_graph / ngia_build_graph). */
 #define NGIA_GRAPH_PLAIN 0u    /* one align launch + host sync per band (buildGraph.hpp:274-350) */
-#define NGIA_GRAPH_BATCHED 1u  /* cross-band dedupe, length-binned kernels, one hipGraph */
+#define NGIA_GRAPH_BATCHED 1u  /* cross-band dedupe, length-binned kernels, captured hipGraphs */
 #define NGIA_GRAPH_BATCHED_EAGER 2u /* the batched kernels launched without the graph, with
                                      * the alignment phase timed (ngia_timing.align_ms) */
 

@@ -265,8 +265,9 @@ void ensure_batched_space(ngia_db* db, uint32_t bands) {
   ensure_edge_space(db, cap);
 }
 
-// The batched round, captured once per (identity, bands) and replayed (or
-// launched eagerly with the alignment phase timed).
+// The batched round: its first part captured once per (identity, bands), its
+// second once per set of bins that hold pairs, both replayed (or launched
+// eagerly, with the alignment phase timed).
 void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
                    ngia_graph_stats* st, bool eager) {
   hipStream_t s = ctx->stream;
