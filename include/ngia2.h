@@ -317,7 +317,10 @@ int ngia_fasta_check(const char* fasta_path);
  * fails the reference's checks (missing, CRLF first line, no final newline). */
 int ngia_makedb(ngia_ctx* ctx, const char* fasta_path, const char* packed_path);
 /* cluster end to end: packed DB -> edges -> Leiden -> result file
- * (main.cu:86-127).  flags as for ngia_db_graph. */
+ * (main.cu:86-127).  flags as for ngia_db_graph.  The batched form goes
+ * through ngia_build_graph_chunked: the whole database when it fits the
+ * device, else blocks of it; env NGIA_BLOCK_WORDS=N forces blocks of N
+ * packed words. */
 int ngia_cluster(ngia_ctx* ctx, const char* packed_path, const char* result_path,
                  uint32_t identity, uint32_t bands, uint32_t flags, ngia_graph_stats* stats);
 

@@ -215,9 +215,13 @@ extern "C" int ngia_cluster(ngia_ctx* ctx, const char* packed_path, const char* 
   if (flags == NGIA_GRAPH_BATCHED) {
     // The whole database on the device when it fits; else blocks of it, the
     // part the reference leaves to managed-memory paging (README.md:8).
+    // NGIA_BLOCK_WORDS=N forces blocks of N packed words (0 or unset: from
+    // free device memory), so the chunked path can be run on a small database.
+    const char* bwe = std::getenv("NGIA_BLOCK_WORDS");
+    const uint64_t bw = bwe ? std::strtoull(bwe, nullptr, 10) : 0;
     ngia_edge* e = nullptr;
     rc = ngia_build_graph_chunked(ctx, pdb.packed, pdb.n_words, pdb.word_off, pdb.lengths, pdb.n,
-                                  pdb.rep_rows, bands, identity, 0, &e, &m, stats, nullptr);
+                                  pdb.rep_rows, bands, identity, bw, &e, &m, stats, nullptr);
     if (rc == NGIA_OK) edges.assign(e, e + m);
     ngia_free(e);
   } else {

@@ -348,6 +348,15 @@ def test_cluster_result_file(ctx, c1db, tmp_path, identity):
     want = str(tmp_path / "w.txt")
     orc.save_result(cl, db, want)
     assert open(res, "rb").read() == open(want, "rb").read()
+    # the same command through the out-of-HBM path: blocks of a sixth of the packed area
+    os.environ["NGIA_BLOCK_WORDS"] = str(len(d["packed"]) // 6)
+    try:
+        say("cluster c1 in blocks, identity", identity)
+        st = ctx.cluster(db, str(tmp_path / "rb.txt"), identity)
+    finally:
+        del os.environ["NGIA_BLOCK_WORDS"]
+    assert st["n_edges"] == len(edges)
+    assert open(str(tmp_path / "rb.txt"), "rb").read() == open(want, "rb").read()
 
 
 @pytest.mark.parametrize("identity", [50, 90])
