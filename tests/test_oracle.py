@@ -253,7 +253,7 @@ def test_fused_and_unfused_band_edges_score_alike():
             orc.align_record(orc.pack(a), orc.pack(b), thr, 0)
 
 
-def test_tile_counts_survey_table():
+def test_tile_counts_equal_lengths():
     # tiles of the reference loop for rep = target = L
     thr = lambda i: float(np.float32(i) / np.float32(100))
     assert orc.align_tiles(300, 300, thr(90)) == 28
