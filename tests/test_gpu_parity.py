@@ -619,17 +619,19 @@ def test_sketch_past_2gib_of_codes(ctx):
         np.testing.assert_array_equal(got[idx], want)
 
 
-@pytest.mark.parametrize("identity", [50, 90])
-def test_chunked_graph_c1(ctx, c1db, identity):
+@pytest.mark.parametrize("identity,bands", [(50, 64), (90, 64), (70, 20)])
+def test_chunked_graph_c1(ctx, c1db, identity, bands):
     """The out-of-HBM form: blocks of about a fifth of the packed area, every
     (rep block, target block) chunk uploaded and aligned alone; the merged
-    edges and the candidate count equal the oracle's."""
+    edges and the candidate count equal the oracle's (64 bands, and the
+    first 20 rows as with -b 20)."""
     _, _, d = c1db
     words = int(d["word_off"][-1]) + 1
-    say("chunked graph c1 identity", identity, "block words", words // 5)
-    got, st, chunks = ctx.build_graph_chunked(d["packed"], d["word_off"], d["lengths"], d["rep"],
-                                              64, identity, block_words=words // 5)
-    want, tot = orc.build_graph(d["packed"], d["word_off"], d["lengths"], d["rep"], 64, identity)
+    rep = np.ascontiguousarray(d["rep"][:bands])
+    say("chunked graph c1 identity", identity, "bands", bands, "block words", words // 5)
+    got, st, chunks = ctx.build_graph_chunked(d["packed"], d["word_off"], d["lengths"], rep,
+                                              bands, identity, block_words=words // 5)
+    want, tot = orc.build_graph(d["packed"], d["word_off"], d["lengths"], rep, bands, identity)
     assert chunks > 5
     assert st["total_pairs"] == tot
     np.testing.assert_array_equal(got, want)
