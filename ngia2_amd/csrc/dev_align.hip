@@ -514,6 +514,10 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
           uint32_t cy = 0;
           #pragma unroll
           for (int m = 0; m < K; m++) s0[m] = __builtin_addc(rows[m], rows[m] & M[m], cy, &cy);
+          // Opaque sums: otherwise the compiler folds the two chains into one,
+          // re-adding the top word and keeping an extra carry (2 VALU per character).
+          #pragma unroll
+          for (int m = 0; m < K; m++) asm("" : "+v"(s0[m]));
           uint32_t all = s0[0];
           #pragma unroll
           for (int m = 1; m < K; m++) all &= s0[m];
