@@ -521,7 +521,14 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
           uint32_t all = s0[0];
           #pragma unroll
           for (int m = 1; m < K; m++) all &= s0[m];
-          const uint64_t G = __builtin_amdgcn_ballot_w64(cy != 0);
+          // G: lanes whose chain carries out.  With one word the carry is
+          // s0 < row (row & M <= ~0, no carry-in), one compare straight into
+          // a lane mask instead of the carry bit kept in a VGPR and tested.
+          uint64_t G;
+          if constexpr (K == 1)
+            G = __builtin_amdgcn_ballot_w64(s0[0] < rows[0]);
+          else
+            G = __builtin_amdgcn_ballot_w64(cy != 0);
           const uint64_t P = __builtin_amdgcn_ballot_w64(all == 0xFFFFFFFFu);
           const uint64_t C = ((G | P) + G) ^ P;  // carry into each lane
           uint32_t ci = __builtin_amdgcn_inverse_ballot_w64(C) ? 1u : 0u;
