@@ -510,31 +510,29 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
         #pragma unroll
         for (int mm = 0; mm < 4; mm++) {
           uint32_t(&M)[K] = Mq[mm];
+          // Both K-word add chains carry through a lane mask (the carry
+          // operand of v_add_co / v_addc_co): the first chain's carry-out is
+          // the lane's generate bit G as it stands, and the lookahead's
+          // carries C enter the second chain as its carry-in mask, so no
+          // carry is kept per lane in a VGPR and tested around the ballots.
+          // The asm also keeps the compiler from folding the two chains.
           uint32_t s0[K];
-          uint32_t cy = 0;
+          uint64_t cy;
+          asm("v_add_co_u32 %0, %1, %2, %3" : "=v"(s0[0]), "=s"(cy) : "v"(rows[0]), "v"(rows[0] & M[0]));
           #pragma unroll
-          for (int m = 0; m < K; m++) s0[m] = __builtin_addc(rows[m], rows[m] & M[m], cy, &cy);
-          // Opaque sums: otherwise the compiler folds the two chains into one,
-          // re-adding the top word and keeping an extra carry (2 VALU per character).
-          #pragma unroll
-          for (int m = 0; m < K; m++) asm("" : "+v"(s0[m]));
+          for (int m = 1; m < K; m++)
+            asm("v_addc_co_u32 %0, %1, %2, %3, %4"
+                : "=v"(s0[m]), "=s"(cy) : "v"(rows[m]), "v"(rows[m] & M[m]), "s"(cy));
           uint32_t all = s0[0];
           #pragma unroll
           for (int m = 1; m < K; m++) all &= s0[m];
-          // G: lanes whose chain carries out.  With one word the carry is
-          // s0 < row (row & M <= ~0, no carry-in), one compare straight into
-          // a lane mask instead of the carry bit kept in a VGPR and tested.
-          uint64_t G;
-          if constexpr (K == 1)
-            G = __builtin_amdgcn_ballot_w64(s0[0] < rows[0]);
-          else
-            G = __builtin_amdgcn_ballot_w64(cy != 0);
+          const uint64_t G = cy;  // lanes whose chain carries out
           const uint64_t P = __builtin_amdgcn_ballot_w64(all == 0xFFFFFFFFu);
-          const uint64_t C = ((G | P) + G) ^ P;  // carry into each lane
-          uint32_t ci = __builtin_amdgcn_inverse_ballot_w64(C) ? 1u : 0u;
+          uint64_t ci = ((G | P) + G) ^ P;  // carry into each lane
           #pragma unroll
           for (int m = 0; m < K; m++) {
-            const uint32_t s = __builtin_addc(s0[m], 0u, ci, &ci);
+            uint32_t s;
+            asm("v_addc_co_u32 %0, %1, %2, 0, %3" : "=v"(s), "=s"(ci) : "v"(s0[m]), "s"(ci));
             rows[m] = s | (rows[m] & ~M[m]);
             asm("" : "+v"(rows[m]));  // as in lcs_step: one bitop3, not two
           }
