@@ -68,6 +68,12 @@ def kernel_name(b):
     return "k_align_plain"
 
 
+TRAFFIC_SOURCE = ("not measured in this run: HBM bytes per launch from the committed "
+                  "profiles/traffic.json, two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) of "
+                  "the same workload scaled by a 1 GiB copy; null when the kernel or workload "
+                  "has no entry there")
+
+
 def load_traffic():
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
     scaled by a plain copy of known size), keyed by kernel and workload."""
@@ -124,6 +130,7 @@ def measure(ng, np, ctx, info, d, identity, steps, warmup, plain_steps, label):
     out = {}
     dt, rec = run(ng.GRAPH_BATCHED, steps, warmup)
     log("%s batched: %.2f ms/step" % (label, 1e3 * dt / steps))
+    edges = db.edges()  # the captured form's sorted unique edge list of the last step
     mean = lambda rs, k: float(np.mean([r[k] for r in rs]))
     out["dt"], out["steps"] = dt, steps
     out["value"] = d.n * steps / dt
@@ -141,28 +148,65 @@ def measure(ng, np, ctx, info, d, identity, steps, warmup, plain_steps, label):
     for i, b in enumerate(bins):
         b["ms"] = float(np.mean([r["bins"][i]["ms"] for r in erec]))
     out["bins"] = bins
-    assert erec[-1]["stats"]["n_edges"] == out["stats"]["n_edges"]
+    out["eager_edges_identical"] = bool(np.array_equal(db.edges(), edges))
+    out["edges"] = edges
 
     if plain_steps:
         pdt, prec = run(ng.GRAPH_PLAIN, plain_steps, 1)
         log("%s plain: %.2f ms/step" % (label, 1e3 * pdt / plain_steps))
-        assert prec[-1]["stats"]["n_edges"] == out["stats"]["n_edges"]
         out["plain"] = {"value": round(d.n * plain_steps / pdt, 1),
                         "ms_per_step": round(1e3 * pdt / plain_steps, 3),
                         "steps": plain_steps,
                         "graph_ms": round(mean(prec, "graph_ms"), 3),
                         "host_syncs": prec[-1]["stats"]["host_syncs"],
                         "align_launches": prec[-1]["stats"]["align_launches"],
-                        "edges_identical": True}
+                        "edges_identical": bool(np.array_equal(db.edges(), edges))}
     db.close()
     return out
 
 
-def serving_bench(ng, np, ctx, d, identity, n_queries, seed, recluster_ms_per_seq):
+def offline_recluster(ng, params, q, identity):
+    """Offline re-clustering of residents + queries, measured: the residents'
+    FASTA (the same generator) with the queries appended, through the command
+    line (makeDB, then cluster with the graph on the GPU, Leiden and the result
+    file on the host)."""
+    import shutil
+    import subprocess
+    import tempfile
+    tmp = tempfile.mkdtemp(prefix="ngia2_recluster_")
+    try:
+        fa, db, res = (os.path.join(tmp, x) for x in ("all.fa", "all.packed", "out.txt"))
+        n_res = ng.gen_fasta(params, fa)
+        with open(fa, "ab") as f:
+            for k in range(q.n):
+                f.write(b">query%d\n" % k + q.seq(k) + b"\n")
+        exe = os.path.join(ng.BIN_DIR, "ngia2")
+        t0 = time.perf_counter()
+        r1 = subprocess.run([exe, "-m", "makedb", "-f", fa, "-p", db], capture_output=True,
+                            text=True, timeout=600)
+        t1 = time.perf_counter()
+        r2 = subprocess.run([exe, "-m", "clustering", "-p", db, "-r", res, "-i", str(identity)],
+                            capture_output=True, text=True, timeout=600)
+        t2 = time.perf_counter()
+        if r1.returncode or r2.returncode:
+            raise RuntimeError("re-clustering failed: %s %s" % (r1.stdout[-400:], r2.stdout[-400:]))
+        with open(res, "rb") as f:
+            clusters = sum(line.startswith(b">") for line in f)
+        return {"sequences": n_res + q.n, "makedb_s": round(t1 - t0, 3),
+                "cluster_s": round(t2 - t1, 3), "ms": round(1e3 * (t2 - t0), 1),
+                "clusters": clusters,
+                "basis": "measured: makeDB + cluster command lines over residents + queries "
+                         "(two processes, files on local disk)"}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def serving_bench(ng, np, ctx, d, params, identity, n_queries, seed):
     """Serving over the resident set: cluster it offline (batched graph + host
     Leiden), then assign a stream of query batches (70% mutated residents at
     0-30% divergence, 30% novel; batch sizes log-uniform over 1..4096) with
-    Poisson arrivals at several offered loads."""
+    Poisson arrivals at several offered loads.  Every served query joins the
+    snapshot, so each run starts a new session (reset) over the residents."""
     t0 = time.perf_counter()
     db = ng.Database(ctx, d.packed, d.word_off, d.lengths)
     db.sketch()
@@ -189,6 +233,7 @@ def serving_bench(ng, np, ctx, d, identity, n_queries, seed, recluster_ms_per_se
     first = np.concatenate([[0], np.cumsum(count)[:-1]])
     sv = ng.Serve(db, part, identity, new_base)
     sv.assign(q)  # warm-up
+    sv.reset()
     t0 = time.perf_counter()
     offline, st_off = sv.assign(q)
     t_off = time.perf_counter() - t0
@@ -201,6 +246,7 @@ def serving_bench(ng, np, ctx, d, identity, n_queries, seed, recluster_ms_per_se
         rate = rho * sat_qps / float(count.mean())  # requests per second
         arrival = np.cumsum(rng.exponential(1.0 / rate, len(count)))
         arrival -= arrival[0]
+        sv.reset()
         got, lat, st = sv.replay(q, arrival, first, count)
         same = bool(np.array_equal(got, offline))
         identical &= same
@@ -228,14 +274,14 @@ def serving_bench(ng, np, ctx, d, identity, n_queries, seed, recluster_ms_per_se
            "offline_assign_ms": round(1e3 * t_off, 3),
            "loads": loads, "assignments_identical_to_offline": identical,
            "new_clusters": int((offline["resident"] == ng.NO_RESIDENT).sum()),
+           "joined_earlier_queries": int(((offline["resident"] != ng.NO_RESIDENT) &
+                                          (offline["resident"] >= d.n)).sum()),
            "mutated_joined": round(float((offline["resident"][kinds] != ng.NO_RESIDENT).mean()), 4)
            if kinds.any() else None,
            "resident_clustering_s": {"graph": round(t_graph, 3), "leiden": round(t_leiden, 3)},
-           "offline_recluster": {
-               "sequences": d.n + q.n,
-               "ms": round(recluster_ms_per_seq * (d.n + q.n), 3),
-               "basis": "the batched hot path's measured time per sequence on the resident set, "
-                        "times residents + queries"}}
+           "offline_recluster": offline_recluster(ng, params, q, identity)}
+    out["offline_recluster"]["serving_speedup"] = round(
+        out["offline_recluster"]["ms"] / (1e3 * t_off), 2)
     return out
 
 
@@ -301,10 +347,12 @@ def end_to_end(ng, config, scale, seed, identity):
         t2 = time.perf_counter()
         if r1.returncode or r2.returncode:
             raise RuntimeError("command line failed: %s %s" % (r1.stdout[-400:], r2.stdout[-400:]))
-        clusters = None
-        for line in r2.stdout.splitlines():
-            if line.startswith("clusters:"):
-                clusters = int(line.split()[-1])
+        # one cluster per representative line (">name" at the line start;
+        # members are written indented, saveResult.hpp:31-83)
+        clusters = 0
+        with open(res, "rb") as f:
+            for line in f:
+                clusters += line.startswith(b">")
         return {"sequences": n, "fasta_bytes": os.path.getsize(fa),
                 "makedb_s": round(t1 - t0, 3), "cluster_s": round(t2 - t1, 3),
                 "seqs_per_s": round(n / (t2 - t0), 1),
@@ -324,9 +372,53 @@ def cpu_baseline(ng, orc, config, scale, seed, identity):
     edges, tot = orc.build_graph(d.packed, d.word_off, d.lengths, rep, 64, identity, 1, threads)
     dt = time.perf_counter() - t0
     return {"value": round(d.n / dt, 1), "unit": "seqs/s", "cores": threads, "kind": "port",
+            "cpu_model": host_cpu(),
             "sample": "the same generator scaled to %d sequences (%g of the families), seed %d, "
                       "-i %d: sketch + bucket + 64-band graph in %.1f s, %d candidate pairs, "
                       "%d edges" % (d.n, scale, seed, identity, dt, tot, len(edges))}
+
+
+def host_cpu():
+    """Model name of the host CPU (lscpu's "Model name", read from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def config1_cpu_build(ng, np, orc, c1_gpu):
+    """Config 1 at full size: the reference algorithm's CPU build (the oracle,
+    all host threads; Leiden single-threaded, seed 42) beside the GPU path
+    measured above, with both edge lists and both partitions compared."""
+    threads = len(os.sched_getaffinity(0))
+    d = c1_gpu["data"]
+    t0 = time.perf_counter()
+    sig = orc.sketch_all(d.packed, d.word_off, threads)
+    rep = orc.bucket(sig, threads)
+    t_sb = time.perf_counter() - t0
+    res = {"cores": threads, "cpu_model": host_cpu(), "kind": "port",
+           "sketch_bucket_s": round(t_sb, 4), "by_identity": {}}
+    for ident in sorted(k for k in c1_gpu if k != "data"):
+        edges, part, gpu_ms = c1_gpu[ident]
+        t0 = time.perf_counter()
+        want, tot = orc.build_graph(d.packed, d.word_off, d.lengths, rep, 64, ident, 1, threads)
+        t_g = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        cl = orc.leiden(want, d.n, orc.resolution(ident))
+        t_l = time.perf_counter() - t0
+        res["by_identity"][str(ident)] = {
+            "graph_s": round(t_g, 4), "leiden_s": round(t_l, 4),
+            "hot_path_seqs_per_s": round(d.n / (t_sb + t_g), 1),
+            "gpu_hot_path_seqs_per_s": round(d.n / (gpu_ms * 1e-3), 1),
+            "edges": int(len(want)), "candidates_with_duplicates": int(tot),
+            "edges_identical": bool(np.array_equal(edges, want)),
+            "clusters": int(cl.max()) + 1 if d.n else 0,
+            "clusters_identical": bool(np.array_equal(part, cl))}
+    return res
 
 
 def main():
@@ -335,9 +427,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=4, help="generator config of the workload")
-    ap.add_argument("--also-configs", default="2:90,3:97",
-                    help="further workloads reported beside the first, config:identity,... "
-                         "('' for none)")
+    ap.add_argument("--also-configs", default="1:50/70/90,2:50/70/90,3:97",
+                    help="further workloads reported beside the first, "
+                         "config:identity[/identity...],... ('' for none)")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--identity", type=int, default=90)
@@ -403,6 +495,7 @@ def main():
     sketch_roof = {"kernel": "k_sketch", "bound": "valu", "achieved": round(sk_ach, 3),
                    "peak": round(peak, 3), "unit": "TOP/s int32", "frac": round(sk_ach / peak, 4),
                    "traffic": traffic.get("k_sketch", {}).get(workload),
+                   "traffic_source": TRAFFIC_SOURCE,
                    "kernel_ms": round(sk_ms, 4), "algorithmic_ops": ops,
                    "algorithmic_bytes": nbytes,
                    "hbm_gbps": round(nbytes / (sk_ms * 1e-3) / 1e9, 2),
@@ -415,6 +508,7 @@ def main():
         roofline = {"kernel": a["kernel"], "bound": "valu", "achieved": a["achieved"],
                     "peak": round(peak, 3), "unit": "TOP/s int32", "frac": a["frac"],
                     "traffic": traffic.get(a["kernel"], {}).get(workload),
+                    "traffic_source": TRAFFIC_SOURCE,
                     "kernel_ms": a["kernel_ms"], "algorithmic_ops": a["ref_tiles"] * OPS_PER_TILE,
                     "ref_tiles": a["ref_tiles"], "pairs": a["pairs"]}
 
@@ -428,7 +522,20 @@ def main():
                       "identity": args.identity, "bands": 64,
                       "residues": int(d.lengths.astype(np.int64).sum()),
                       "max_length": int(d.lengths.max()), "seed": args.seed,
+                      "length_model": {"body": "log-normal", "median": params.len_a,
+                                       "sigma": params.len_b, "min": params.len_min,
+                                       "max": params.len_max,
+                                       "tail_fraction_of_families": params.tail_frac,
+                                       "tail_x_min": params.tail_xmin,
+                                       "tail_alpha": params.tail_alpha,
+                                       "tail_max": params.tail_max},
+                      "divergence": {"substitutions": [params.sub_lo, params.sub_hi],
+                                     "indels": [params.indel_lo, params.indel_hi]},
                       "form": "batched (length-binned kernels, captured hipGraph)"},
+           "value_covers": "sequences per second through the device hot path: sketch, band "
+                           "bucketing and the graph up to the sorted unique edge list on the "
+                           "device, database resident in HBM; host Leiden and the files are "
+                           "outside it (end_to_end times the command line with them)",
            "aligned_pairs_per_s": round(st["total_pairs"] / ((m["graph_ms"] + m["finalize_ms"])
                                                               * 1e-3), 1),
            "distinct_pairs_per_s": round(st["distinct_pairs"] / ((m["graph_ms"] + m["finalize_ms"])
@@ -457,44 +564,60 @@ def main():
             out["out_of_hbm"]["edges_identical"]))
 
     if args.serve_queries:
-        sv = serving_bench(ng, np, ctx, d, args.identity, args.serve_queries, args.seed + 100,
-                           m["ms_per_step"] / d.n)
+        sv = serving_bench(ng, np, ctx, d, params, args.identity, args.serve_queries,
+                           args.seed + 100)
         out["serving"] = sv
         mid = sv["loads"][1]
         out["latency_ms"] = {"p50": mid["p50_ms"], "p99": mid["p99_ms"],
                              "offered_load": mid["offered_load"],
                              "queries_per_s": mid["queries_per_s"]}
 
+    c1_gpu = {}
     for spec in [x for x in args.also_configs.split(",") if x]:
-        cfg, ident = (int(v) for v in spec.split(":"))
+        cfg, idents = spec.split(":")
+        cfg = int(cfg)
         p2 = ng.gen_config(cfg, args.scale, args.seed)
         d2 = ng.gen_packed(p2)
-        m2 = measure(ng, np, ctx, info, d2, ident, args.steps, args.warmup,
-                     0 if args.no_plain else (3 if cfg != 3 else 1), "config %d" % cfg)
-        rows2, phase2 = align_rooflines(m2, peak)
         kind = "DNA (ACGT)" if p2.dna else "proteins"
         lens = ("normal lengths (mean %g, sd %g, %d-%d)" % (p2.len_a, p2.len_b, p2.len_min, p2.len_max)
                 if p2.len_normal else "log-normal lengths (median %g, max %d)" % (p2.len_a, p2.len_max))
-        o2 = {"workload": "%d %s in %d families, %s, -i %d" % (d2.n, kind, p2.n_families, lens, ident),
-              "identity": ident, "value": round(m2["value"], 1),
-              "ms_per_step": round(m2["ms_per_step"], 3),
-              "residues": int(d2.lengths.astype(np.int64).sum()),
-              "stage_ms": {"sketch": round(m2["sketch_ms"], 3), "bucket": round(m2["bucket_ms"], 3),
-                           "graph": round(m2["graph_ms"], 3),
-                           "finalize": round(m2["finalize_ms"], 3)},
-              "pairs": {"candidates_with_duplicates": m2["stats"]["total_pairs"],
-                        "distinct": m2["stats"]["distinct_pairs"],
-                        "edges": m2["stats"]["n_edges"], "ref_tiles": m2["stats"]["ref_tiles"]},
-              "aligned_pairs_per_s": round(m2["stats"]["total_pairs"] /
-                                           ((m2["graph_ms"] + m2["finalize_ms"]) * 1e-3), 1),
-              "align_phase": phase2, "align_kernels": rows2[:4]}
-        ops2, nbytes2 = sketch_work(d2.lengths)
-        o2["sketch_hbm_gbps"] = round(nbytes2 / (m2["sketch_ms"] * 1e-3) / 1e9, 2)
-        if "plain" in m2:
-            o2["plain_form"] = m2["plain"]
-            o2["batched_vs_plain"] = round(m2["value"] / m2["plain"]["value"], 3)
+        o2 = {"workload": "%d %s in %d families, %s" % (d2.n, kind, p2.n_families, lens),
+              "residues": int(d2.lengths.astype(np.int64).sum()), "by_identity": {}}
+        for ident in (int(v) for v in idents.split("/")):
+            plain = 0 if args.no_plain else (3 if cfg != 3 else 1)
+            m2 = measure(ng, np, ctx, info, d2, ident, args.steps, args.warmup, plain,
+                         "config %d -i %d" % (cfg, ident))
+            rows2, phase2 = align_rooflines(m2, peak)
+            r = {"identity": ident, "value": round(m2["value"], 1),
+                 "ms_per_step": round(m2["ms_per_step"], 3),
+                 "stage_ms": {"sketch": round(m2["sketch_ms"], 3),
+                              "bucket": round(m2["bucket_ms"], 3),
+                              "graph": round(m2["graph_ms"], 3),
+                              "finalize": round(m2["finalize_ms"], 3)},
+                 "pairs": {"candidates_with_duplicates": m2["stats"]["total_pairs"],
+                           "distinct": m2["stats"]["distinct_pairs"],
+                           "edges": m2["stats"]["n_edges"], "ref_tiles": m2["stats"]["ref_tiles"]},
+                 "aligned_pairs_per_s": round(m2["stats"]["total_pairs"] /
+                                              ((m2["graph_ms"] + m2["finalize_ms"]) * 1e-3), 1),
+                 "align_phase": phase2, "align_kernels": rows2[:4],
+                 "eager_edges_identical": m2["eager_edges_identical"]}
+            ops2, nbytes2 = sketch_work(d2.lengths)
+            r["sketch_hbm_gbps"] = round(nbytes2 / (m2["sketch_ms"] * 1e-3) / 1e9, 2)
+            if "plain" in m2:
+                r["plain_form"] = m2["plain"]
+                r["batched_vs_plain"] = round(m2["value"] / m2["plain"]["value"], 3)
+            if cfg == 1:
+                t0 = time.perf_counter()
+                part = ng.leiden(m2["edges"], d2.n, ng.resolution(ident))
+                r["leiden_s"] = round(time.perf_counter() - t0, 4)
+                r["clusters"] = int(part.max()) + 1 if d2.n else 0
+                c1_gpu[ident] = (m2["edges"], part, m2["ms_per_step"])
+            o2["by_identity"][str(ident)] = r
         out["config_%d" % cfg] = o2
-        del d2
+        if cfg == 1:
+            c1_gpu["data"] = d2
+        else:
+            del d2
 
     ctx.close()
     if not args.no_e2e:
@@ -507,6 +630,10 @@ def main():
         out["cpu_baseline"] = cpu_baseline(ng, orc, args.config, args.cpu_scale, args.seed,
                                            args.identity)
         log("cpu baseline: %.1f seqs/s" % out["cpu_baseline"]["value"])
+        if "data" in c1_gpu:
+            out["config_1"]["cpu_build"] = config1_cpu_build(ng, np, orc, c1_gpu)
+            log("config 1 cpu build: %s" % {k: (v["edges_identical"], v["clusters_identical"])
+                                            for k, v in out["config_1"]["cpu_build"]["by_identity"].items()})
     print(json.dumps(out), flush=True)
 
 

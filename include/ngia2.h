@@ -176,6 +176,17 @@ int ngia_db_get_edges(ngia_ctx* ctx, ngia_db* db, ngia_edge* out, uint64_t cap,
 int ngia_db_get_pair_scores(ngia_ctx* ctx, ngia_db* db, uint64_t* keys, uint32_t* scores,
                             uint64_t cap, uint64_t* n_pairs);
 
+/* One band of the plain form (buildGraph.hpp:274-333): the band's job
+ * triplets (rep[i], i, score) for ascending i with rep[i] != i and the isJob
+ * length test (buildGraph.hpp:262-300, 305-321), scored by the kernel_dynamic
+ * contract (buildGraph.hpp:192-253), in the order the reference's jobs array
+ * holds them after the launch.  *count receives the band's job count (at most
+ * n); jobs (3 * count uint32, may be NULL) receives the triplets when cap >=
+ * count.  Needs representative rows for the band (ngia_db_bucket or
+ * ngia_db_set_reps). */
+int ngia_db_band_jobs(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t band,
+                      uint32_t* jobs, uint64_t cap, uint64_t* count);
+
 /* Length bins of the last batched graph.  No reference counterpart (the
  * reference aligns every band's jobs in one kernel_dynamic launch).  A pair's
  * bin follows from its band window W = lsft/32 + ceil(rsft/32) words:
@@ -236,37 +247,49 @@ int ngia_db_fast(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
 /* No reference counterpart: the reference only clusters offline.  A serving
  * handle keeps a clustered database resident -- packed records, byte codes,
  * the 64 signatures of every sequence and its cluster id -- and assigns
- * streamed query sequences to clusters.
+ * streamed query sequences to clusters.  The snapshot grows with the stream:
+ * the session's p-th query (counted in arrival order over every assign and
+ * replay call since creation or the last ngia_serve_reset) becomes snapshot
+ * index n + p, so later queries can join it.
  *
- * Assignment of one query (fixed by the resident snapshot alone, so it is the
- * same whether the query is served alone, in a mixed batch or offline):
+ * Assignment of one query (fixed by the residents and the session's earlier
+ * queries alone, so it is the same whether the query is served alone, in a
+ * mixed batch or offline):
  *  1. sketch the query as makeDB sketches a record (makedb.hpp:224-247);
- *  2. per band, the candidate is the bucket head of the query's signature:
- *     the lowest resident index with that band signature (the row that
- *     groupSignatures makes every member point back to, makedb.hpp:340-347);
+ *  2. per band, the candidate is the bucket head of the query's signature
+ *     among the residents and the earlier queries: the lowest snapshot index
+ *     with that band signature (the row that groupSignatures makes every
+ *     member point back to, makedb.hpp:340-347), when there is one;
  *  3. each distinct candidate forms a pair with the longer sequence as rep
- *     (ties: the resident), kept if (float)len[tar] >= (float)len[rep]*thr
+ *     (ties: the candidate), kept if (float)len[tar] >= (float)len[rep]*thr
  *     (the isJob test, buildGraph.hpp:262-266), and is scored by the
  *     kernel_dynamic recurrence (buildGraph.hpp:192-253);
- *  4. the query joins cluster_of[r] of the best-scoring resident r (highest
- *     score, then lowest index); with no passing pair it opens a new
- *     cluster numbered new_base + its query id.
+ *  4. the query joins the cluster of its best candidate (highest score, then
+ *     lowest index): cluster_of[r] for a resident r, the cluster assigned to
+ *     an earlier query otherwise; with no passing pair it opens a new cluster
+ *     numbered new_base + its query id.
  * Requests wait in arrival order and are admitted into batches as lanes free
- * up; a lane's capacity in queries and residues is fixed at creation from
- * the free HBM.  Lanes run on their own streams, so one batch's host packing
- * and copies overlap another's kernels. */
+ * up; a batch takes at most max_queries queries and max_residues residues,
+ * and a lane's device buffers grow toward max_queries at admission only
+ * while its share (hbm_fraction / lanes) of the free HBM at that moment
+ * allows -- else the batch is cut to what the lane holds.  Lanes run on their
+ * own streams, so one batch's host packing and copies overlap another's
+ * kernels; batches join the snapshot in arrival order. */
 #define NGIA_NO_RESIDENT 0xFFFFFFFFu
 typedef struct {
-  uint32_t cluster;     /* cluster_of[resident], or new_base + query id */
-  uint32_t resident;    /* best resident index, NGIA_NO_RESIDENT when a new cluster opens */
+  uint32_t cluster;     /* the best candidate's cluster, or new_base + query id */
+  uint32_t resident;    /* snapshot index of the best candidate (>= n: the session's query
+                         * resident - n); NGIA_NO_RESIDENT when a new cluster opens */
   uint32_t score;       /* its score 1..100, 0 for a new cluster */
-  uint32_t candidates;  /* distinct residents aligned (after the length test) */
+  uint32_t candidates;  /* distinct candidates aligned (after the length test) */
 } ngia_assignment;
 typedef struct {
   uint32_t lanes;          /* batches in flight at once (streams); 0 -> 2 */
   uint32_t max_queries;    /* queries per batch; 0 -> 8192 */
-  uint64_t max_residues;   /* residues per batch; 0 -> from free HBM, at most 2^26 */
-  double hbm_fraction;     /* share of the free HBM all lanes may take; 0 -> 0.25 */
+  uint64_t max_residues;   /* residues per batch; 0 -> 2^24 */
+  double hbm_fraction;     /* share of the free HBM the lanes may grow into; 0 -> 0.25 */
+  uint32_t grow_queries;   /* queries a session can add to the snapshot; 0 -> 2^20 */
+  uint64_t grow_residues;  /* residues a session can add; 0 -> 2^28 */
 } ngia_serve_config;
 typedef struct {
   double arrival;          /* seconds after the start of the replay (non-decreasing) */
@@ -278,15 +301,22 @@ typedef struct {
   uint64_t max_residues;
   double wall_s;           /* start of the replay to the last completion */
   double device_s;         /* summed device time of the batches (HIP events) */
+  uint64_t grown;          /* queries in the snapshot's grown part after the call */
 } ngia_serve_stats;
 /* Resident snapshot from db (which must be sketched: ngia_db_sketch) and
- * cluster_of[db n]; identity 1..99. */
+ * cluster_of[db n]; identity 1..99.  The snapshot and room for the grown part
+ * are allocated here (NGIA_ERR_NOMEM when they exceed the free HBM). */
 int ngia_serve_create(ngia_ctx* ctx, ngia_db* db, const uint32_t* cluster_of, uint32_t identity,
                       uint32_t new_base, const ngia_serve_config* cfg, ngia_serve** out);
 void ngia_serve_destroy(ngia_serve* sv);
+/* Start a new session: the snapshot returns to the residents alone.  A call
+ * that would take the grown part past grow_queries or grow_residues fails
+ * with NGIA_ERR_NOMEM (the batches admitted before stay in the snapshot). */
+int ngia_serve_reset(ngia_serve* sv);
 /* Queries are residue strings (FASTA letters): query q is
  * res[res_off[q] .. res_off[q] + res_len[q]), res_len[q] < 65535; its query id
- * is first_qid + q. */
+ * is first_qid + q (ids name new clusters: keep them distinct within a
+ * session). */
 int ngia_serve_assign(ngia_serve* sv, const uint8_t* res, const uint64_t* res_off,
                       const uint32_t* res_len, uint32_t n_queries, uint64_t first_qid,
                       ngia_assignment* out, ngia_serve_stats* stats);

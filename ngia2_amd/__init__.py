@@ -274,6 +274,15 @@ class Database:
                                              C.byref(m)), self.ctx.h)
         return keys, sc
 
+    def band_jobs(self, identity, band):
+        """The plain form's scored job triplets of one band (ngia_db_band_jobs):
+        (count, 3) uint32 rows (rep, tar, score) in the reference's job order."""
+        jobs = np.zeros((max(self.n, 1), 3), dtype=np.uint32)
+        c = C.c_uint64()
+        _check(_lib.lib().ngia_db_band_jobs(self.ctx.h, self.h, identity, band, _ptr(jobs),
+                                            self.n, C.byref(c)), self.ctx.h)
+        return jobs[:c.value].copy()
+
     def bin_stats(self):
         """Length bins of the last batched graph: kind, width, pairs, ref_tiles, ms."""
         L = _lib.lib()
@@ -321,11 +330,12 @@ class Serve:
     """A resident clustered snapshot serving query batches (ngia_serve_*)."""
 
     def __init__(self, db, cluster_of, identity, new_base, lanes=0, max_queries=0,
-                 max_residues=0, hbm_fraction=0.0):
+                 max_residues=0, hbm_fraction=0.0, grow_queries=0, grow_residues=0):
         self.ctx = db.ctx
         self._cluster_of = _u32(cluster_of)
         assert len(self._cluster_of) == db.n
-        cfg = _lib.ServeConfig(lanes, max_queries, max_residues, hbm_fraction)
+        cfg = _lib.ServeConfig(lanes, max_queries, max_residues, hbm_fraction, grow_queries,
+                               grow_residues)
         h = C.c_void_p()
         _check(_lib.lib().ngia_serve_create(self.ctx.h, db.h, _ptr(self._cluster_of), identity,
                                             new_base, C.byref(cfg), C.byref(h)), self.ctx.h)
@@ -342,6 +352,10 @@ class Serve:
             self.close()
         except Exception:
             pass
+
+    def reset(self):
+        """A new session: the snapshot returns to the residents (ngia_serve_reset)."""
+        _check(_lib.lib().ngia_serve_reset(self.h), self.ctx.h)
 
     def assign(self, queries, first_qid=0):
         """All queries at once (the offline run): structured array of ASSIGN_DTYPE."""

@@ -123,7 +123,8 @@ class Assignment(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("lanes", C.c_uint32), ("max_queries", C.c_uint32),
-                ("max_residues", C.c_uint64), ("hbm_fraction", C.c_double)]
+                ("max_residues", C.c_uint64), ("hbm_fraction", C.c_double),
+                ("grow_queries", C.c_uint32), ("grow_residues", C.c_uint64)]
 
 
 class Request(C.Structure):
@@ -134,7 +135,7 @@ class ServeStats(C.Structure):
     _fields_ = [("queries", C.c_uint64), ("requests", C.c_uint64), ("batches", C.c_uint64),
                 ("pairs", C.c_uint64), ("new_clusters", C.c_uint64), ("lanes", C.c_uint32),
                 ("max_queries", C.c_uint32), ("max_residues", C.c_uint64),
-                ("wall_s", C.c_double), ("device_s", C.c_double)]
+                ("wall_s", C.c_double), ("device_s", C.c_double), ("grown", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -183,6 +184,7 @@ PROTOTYPES = {
     "ngia_db_graph": (C.c_int, [P, P, U32, U32, U32, C.POINTER(GraphStats)]),
     "ngia_db_get_edges": (C.c_int, [P, P, P, U64, C.POINTER(U64)]),
     "ngia_db_get_pair_scores": (C.c_int, [P, P, P, P, U64, C.POINTER(U64)]),
+    "ngia_db_band_jobs": (C.c_int, [P, P, U32, U32, P, U64, C.POINTER(U64)]),
     "ngia_db_bin_stats": (C.c_int, [P, P, P, U32, C.POINTER(U32)]),
     "ngia_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
     "ngia_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
@@ -190,6 +192,7 @@ PROTOTYPES = {
     "ngia_db_fast": (C.c_int, [P, P, U32, U32, P, C.POINTER(U64)]),
     "ngia_serve_create": (C.c_int, [P, P, P, U32, U32, C.POINTER(ServeConfig), C.POINTER(P)]),
     "ngia_serve_destroy": (None, [P]),
+    "ngia_serve_reset": (C.c_int, [P]),
     "ngia_serve_assign": (C.c_int, [P, P, P, P, U32, U64, P, C.POINTER(ServeStats)]),
     "ngia_serve_replay": (C.c_int, [P, P, P, P, U32, P, U32, P, P, C.POINTER(ServeStats)]),
     "ngia_leiden": (C.c_int, [P, U64, I32, C.c_float, U32, I32, P]),

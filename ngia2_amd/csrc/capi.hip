@@ -44,42 +44,60 @@ void finalize(ngia_ctx* ctx, ngia_db* db, uint64_t m) {
   db->n_edges = u;
 }
 
+// Workspace of the plain form: per-band flags, scan, job triplets and the
+// alignment row scratch.
+void ensure_plain_space(ngia_db* db) {
+  const uint32_t n = db->n;
+  db->flags.ensure(4ull * n);
+  db->pos.ensure(8ull * (n + 1));
+  db->sums.ensure(8 * scan_scratch_words(n));
+  db->jobs.ensure(12ull * n);
+  db->plain_scratch.ensure(4 * align_plain_scratch_words((db->max_len + 31) / 32));
+}
+
+// One band of the plain form up to the alignment (buildGraph.hpp:274-333):
+// the band's job triplets (rep[i], i, 0) for ascending i with isJob, their
+// count read back by the host (a stream synchronisation), then one
+// kernel_dynamic-contract launch scoring them in place.  Returns the count.
+uint64_t plain_band(ngia_ctx* ctx, ngia_db* db, uint32_t band, float thr, const uint32_t* parent,
+                    float* ms_cand) {
+  hipStream_t s = ctx->stream;
+  const uint32_t n = db->n;
+  const uint32_t* rep = db->rep.as<uint32_t>() + (uint64_t)band * n;
+  NGIA_HIP(hipEventRecord(ctx->ev[0], s));
+  band_jobs(rep, db->lengths.as<uint32_t>(), n, thr, parent, db->flags.as<uint32_t>(),
+            db->pos.as<uint64_t>(), db->sums.as<uint64_t>(), db->jobs.as<uint32_t>(), s);
+  uint64_t count = 0;
+  d2h(&count, db->pos.as<uint64_t>() + n, 8, s);
+  NGIA_HIP(hipEventRecord(ctx->ev[1], s));
+  NGIA_HIP(hipStreamSynchronize(s));  // the host needs the count to size the launch
+  if (ms_cand) *ms_cand += event_ms(ctx->ev[0], ctx->ev[1]);
+  if (count == 0) return 0;
+  const uint32_t nw_max = (db->max_len + 31) / 32;
+  launch_align_plain(align_args(db, thr), db->jobs.as<uint32_t>(), (uint32_t)count, nw_max,
+                     db->plain_scratch.as<uint32_t>(), align_plain_scratch_words(nw_max), s);
+  return count;
+}
+
 void graph_plain(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
                  ngia_graph_stats* st, const uint32_t* parent /* fast mode */) {
   hipStream_t s = ctx->stream;
   const uint32_t n = db->n;
   const float thr = (float)identity / 100.0f;
   db->have_bins = false;
-  db->flags.ensure(4ull * n);
-  db->pos.ensure(8ull * (n + 1));
-  db->sums.ensure(8 * scan_scratch_words(n));
-  db->jobs.ensure(12ull * n);
+  ensure_plain_space(db);
   db->counters.ensure(sizeof(Counters));
-  const uint32_t nw_max = (db->max_len + 31) / 32;
-  const uint64_t sw = align_plain_scratch_words(nw_max);
-  db->plain_scratch.ensure(4 * sw);
   if (!parent) ensure_edge_space(db, (uint64_t)n * bands);
   fill_u32(db->counters.as<uint32_t>(), 0, sizeof(Counters) / 4, s);
   Counters* dc = db->counters.as<Counters>();
-  const AlignArgs aa = align_args(db, thr);
   uint64_t total = 0;
   float ms_cand = 0, ms_align = 0, ms_edges = 0;
   uint32_t launches = 0, syncs = 0;
   for (uint32_t b = 0; b < bands; b++) {
-    const uint32_t* rep = db->rep.as<uint32_t>() + (uint64_t)b * n;
-    NGIA_HIP(hipEventRecord(ctx->ev[0], s));
-    band_jobs(rep, db->lengths.as<uint32_t>(), n, thr, parent, db->flags.as<uint32_t>(),
-              db->pos.as<uint64_t>(), db->sums.as<uint64_t>(), db->jobs.as<uint32_t>(), s);
-    uint64_t count = 0;
-    d2h(&count, db->pos.as<uint64_t>() + n, 8, s);
-    NGIA_HIP(hipEventRecord(ctx->ev[1], s));
-    NGIA_HIP(hipStreamSynchronize(s));  // the host needs the count to size the launch
+    const uint64_t count = plain_band(ctx, db, b, thr, parent, &ms_cand);
     syncs++;
-    ms_cand += event_ms(ctx->ev[0], ctx->ev[1]);
     if (count == 0) continue;
     total += count;
-    launch_align_plain(aa, db->jobs.as<uint32_t>(), (uint32_t)count, nw_max,
-                       db->plain_scratch.as<uint32_t>(), sw, s);
     launches++;
     NGIA_HIP(hipEventRecord(ctx->ev[2], s));
     NGIA_HIP(hipStreamSynchronize(s));  // buildGraph.hpp:333
@@ -215,7 +233,7 @@ void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
     mark(kBinPlain, 0, s);
     launch_align_plain(aa, db->plain_jobs.as<uint32_t>(), 0xFFFFFFFFu, nw_max,
                        db->plain_scratch.as<uint32_t>(), db->plain_scratch.bytes / 4, s,
-                       &dc->bin_cnt[kBinPlain]);
+                       &dc->bin_cnt[kBinPlain], kPlainBinBlocks);
     mark(kBinPlain, 1, s);
     plain_bin_scores(db->list.as<uint32_t>(), dc->bin_off, dc->bin_cnt,
                      db->plain_jobs.as<uint32_t>(), db->score.as<uint32_t>(), s);
@@ -257,12 +275,21 @@ void ensure_batched_space(ngia_db* db, uint32_t bands) {
   db->score.ensure(4 * cap);
   db->bin_of.ensure(4 * cap);
   db->list.ensure(4 * cap);
-  const uint32_t nw_max = (db->max_len + 31) / 32;
-  // The plain bin only holds windows wider than 2048 words; its job list and
-  // row scratch are sized for the whole pair list so the graph never grows.
-  db->plain_jobs.ensure(12 * cap);
-  db->plain_scratch.ensure(4 * align_plain_scratch_words(nw_max));
   ensure_edge_space(db, cap);
+}
+
+// The plain bin holds only windows wider than 64 x 32 words (records of
+// tens of thousands of residues at low identity), so its job list and row
+// scratch are allocated the first time a round's bin holds pairs, not with
+// the rest of the workspace: the row scratch alone is kPlainBinBlocks x 64
+// threads x ceil(R/32) words.  Returns true when either buffer moved.
+bool ensure_plain_bin_space(ngia_db* db, uint32_t bands) {
+  const void* j0 = db->plain_jobs.p;
+  const void* s0 = db->plain_scratch.p;
+  const uint64_t cap = std::max<uint64_t>((uint64_t)db->n * bands, 1);
+  db->plain_jobs.ensure(12 * cap);
+  db->plain_scratch.ensure(4 * align_plain_scratch_words((db->max_len + 31) / 32, kPlainBinBlocks));
+  return db->plain_jobs.p != j0 || db->plain_scratch.p != s0;
 }
 
 // The batched round: its first part captured once per (identity, bands), its
@@ -278,8 +305,7 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   std::vector<uintptr_t> key;
   for (const DevBuf* b : {&db->packed, &db->word_off, &db->lengths, &db->codes, &db->code_off,
                           &db->rep, &db->counters, &db->cnt, &db->keep, &db->off, &db->sums, &db->pairs,
-                          &db->score, &db->bin_of, &db->list, &db->plain_jobs, &db->plain_scratch,
-                          &db->ekeys, &db->escores})
+                          &db->score, &db->bin_of, &db->list, &db->ekeys, &db->escores})
     key.push_back((uintptr_t)b->p);
   for (uint32_t w : ctx->route.route_below) key.push_back(w);
   const bool reuse = db->gexec && db->g_identity == identity && db->g_bands == bands &&
@@ -303,6 +329,17 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   uint32_t mask = 0;
   for (int b = 0; b < kBins; b++)
     if (cnt[b]) mask |= 1u << b;
+  if (((mask >> kBinPlain) & 1u) && ensure_plain_bin_space(db, bands)) {
+    // captures that launch the plain bin hold the old buffers
+    std::vector<std::pair<uint32_t, hipGraphExec_t>> keep;
+    for (auto& e : db->gexec2) {
+      if ((e.first >> kBinPlain) & 1u)
+        (void)hipGraphExecDestroy(e.second);
+      else
+        keep.push_back(e);
+    }
+    db->gexec2.swap(keep);
+  }
   if (eager) {
     // Same kernels without the graph, with events around the alignment phase.
     enqueue_align(ctx, db, identity, mask, /*time_align=*/true);
@@ -555,6 +592,26 @@ extern "C" int ngia_db_graph(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint
       graph_plain(ctx, db, identity, bands, st, nullptr);
     auto t1 = std::chrono::steady_clock::now();
     if (st) st->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return NGIA_OK;
+  });
+}
+
+extern "C" int ngia_db_band_jobs(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t band,
+                                 uint32_t* jobs, uint64_t cap, uint64_t* count) {
+  if (!ctx || !db || !count) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  if (identity < 1 || identity > 99) return fail(ctx, NGIA_ERR_ARG, "identity should be 1-99");
+  if (band >= db->rep_bands) return fail(ctx, NGIA_ERR_STATE, "no representative row for band");
+  if (db->n == 0) return fail(ctx, NGIA_ERR_ARG, "empty database");
+  *count = 0;
+  return guarded(ctx, [&] {
+    ensure_plain_space(db);
+    db->have_bins = false;
+    const uint64_t c = plain_band(ctx, db, band, (float)identity / 100.0f, nullptr, nullptr);
+    *count = c;
+    if (!jobs || c == 0) return NGIA_OK;
+    if (cap < c) return fail(ctx, NGIA_ERR_ARG, "job buffer too small");
+    d2h(jobs, db->jobs.p, 12 * c, ctx->stream);
+    NGIA_HIP(hipStreamSynchronize(ctx->stream));
     return NGIA_OK;
   });
 }

@@ -135,9 +135,76 @@ __device__ inline uint32_t lcs_step(uint32_t row, uint32_t M, uint32_t cin, uint
 
 // ------------------------------------------------------------- plain form
 // One thread per job triplet, the whole band of any width; the row vector
-// lives in a lane-interleaved global scratch (rows of thread t at
-// scratch[w * stride + t]), the 16 match masks of the current rep word in a
-// lane-interleaved LDS table.  Grid-stride over the jobs.
+// lives in a lane-interleaved global scratch (row w of the thread at
+// rows[w * stride]), the 16 match masks of the current rep word in a
+// lane-interleaved LDS table.  The target's codes come from the byte codes
+// (kPlanes = false) or straight from its four bit planes (kPlanes = true, for
+// callers that hand over the packed area alone).
+template <bool kPlanes>
+__device__ inline void plain_job(const AlignArgs& A, uint32_t* __restrict__ jobs, uint32_t job,
+                                 uint32_t R, uint32_t* __restrict__ rows, uint32_t stride,
+                                 uint32_t* peq, uint32_t lane) {
+  const uint32_t rep = jobs[3 * job], tar = jobs[3 * job + 1];
+  const uint32_t* rr = A.packed + A.word_off[rep];
+  const uint32_t* tr = A.packed + A.word_off[tar];
+  const uint32_t T = tr[0];
+  if (length_rejects(R, T, A.thr)) return;  // score keeps its initial 0
+  const uint32_t nw = (R + 31) / 32;
+  for (uint32_t w = 0; w < nw; w++) rows[(uint64_t)w * stride] = 0xFFFFFFFFu;
+  const Band bd = band_edges(R, T, A.thr);
+  const uint8_t* tc = kPlanes ? nullptr : A.codes + A.code_off[tar];
+  for (uint32_t i = 0; i < T; i += 32) {
+    int32_t js = (int32_t)i - (int32_t)bd.lsft;
+    if (js < 0) js = 0;
+    int32_t je = (int32_t)i + (int32_t)bd.rsft;
+    if (je > (int32_t)R) je = (int32_t)R;
+    uint32_t cw[8];  // codes of characters 4q..4q+3 of the block, one per byte
+    if (kPlanes) {
+      const uint32_t* pl = tr + 1 + (i / 32) * 4;
+      const uint32_t p0 = pl[0], p1 = pl[1], p2 = pl[2], p3 = pl[3];
+      #pragma unroll
+      for (int q = 0; q < 8; q++) {
+        uint32_t w = 0;
+        #pragma unroll
+        for (int m = 0; m < 4; m++) {
+          const uint32_t k = 4 * q + m;
+          w |= (((p0 >> k) & 1u) | (((p1 >> k) & 1u) << 1) | (((p2 >> k) & 1u) << 2) |
+                (((p3 >> k) & 1u) << 3)) << (8 * m);
+        }
+        cw[q] = w;
+      }
+    } else {
+      const uint4 c0 = *(const uint4*)(tc + i), c1 = *(const uint4*)(tc + i + 16);
+      cw[0] = c0.x; cw[1] = c0.y; cw[2] = c0.z; cw[3] = c0.w;
+      cw[4] = c1.x; cw[5] = c1.y; cw[6] = c1.z; cw[7] = c1.w;
+    }
+    uint32_t carrys = 0;  // carry out of the previous word, bit k for char k
+    for (uint32_t j = (uint32_t)js; j < (uint32_t)je; j += 32) {
+      const uint32_t* pl = rr + 1 + (j / 32) * 4;
+      uint32_t m[16];
+      peq_masks(pl[0], pl[1], pl[2], pl[3], m);
+      #pragma unroll
+      for (int k = 0; k < 16; k++) peq[k * 64 + lane] = m[k];
+      uint32_t row = rows[(uint64_t)(j / 32) * stride];
+      uint32_t cout_bits = 0;
+      #pragma unroll
+      for (int k = 0; k < 32; k++) {
+        const uint32_t code = (cw[k >> 2] >> (8 * (k & 3))) & 0xFu;
+        const uint32_t M = peq[code * 64 + lane];
+        uint32_t co;
+        row = lcs_step(row, M, (carrys >> k) & 1u, &co);
+        cout_bits |= co << k;
+      }
+      carrys = cout_bits;
+      rows[(uint64_t)(j / 32) * stride] = row;
+    }
+  }
+  uint32_t lcs = 0;
+  for (uint32_t w = 0; w < nw; w++) lcs += 32u - __popc(rows[(uint64_t)w * stride]);
+  jobs[3 * job + 2] = pair_value(A, lcs, R, T);
+}
+
+// Grid-stride over the jobs; `stride` threads in all.
 __global__ __launch_bounds__(64) void k_align_plain(AlignArgs A, uint32_t* __restrict__ jobs,
                                                     uint32_t count, const uint32_t* count_ptr,
                                                     uint32_t* __restrict__ scratch,
@@ -147,61 +214,20 @@ __global__ __launch_bounds__(64) void k_align_plain(AlignArgs A, uint32_t* __res
   const uint32_t tid = blockIdx.x * 64 + lane;
   if (count_ptr) count = *count_ptr;
   for (uint32_t job = tid; job < count; job += stride) {
-    const uint32_t rep = jobs[3 * job], tar = jobs[3 * job + 1];
-    const uint32_t* rr = A.packed + A.word_off[rep];
-    const uint32_t R = rr[0];
-    const uint32_t T = A.packed[A.word_off[tar]];
-    if (length_rejects(R, T, A.thr)) continue;  // score keeps its initial 0
-    const uint32_t nw = (R + 31) / 32;
-    for (uint32_t w = 0; w < nw; w++) scratch[(uint64_t)w * stride + tid] = 0xFFFFFFFFu;
-    const Band bd = band_edges(R, T, A.thr);
-    const uint8_t* tc = A.codes + A.code_off[tar];
-    for (uint32_t i = 0; i < T; i += 32) {
-      int32_t js = (int32_t)i - (int32_t)bd.lsft;
-      if (js < 0) js = 0;
-      int32_t je = (int32_t)i + (int32_t)bd.rsft;
-      if (je > (int32_t)R) je = (int32_t)R;
-      const uint4 c0 = *(const uint4*)(tc + i), c1 = *(const uint4*)(tc + i + 16);
-      const uint32_t cw[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-      uint32_t carrys = 0;  // carry out of the previous word, bit k for char k
-      for (uint32_t j = (uint32_t)js; j < (uint32_t)je; j += 32) {
-        const uint32_t* pl = rr + 1 + (j / 32) * 4;
-        uint32_t m[16];
-        peq_masks(pl[0], pl[1], pl[2], pl[3], m);
-        #pragma unroll
-        for (int k = 0; k < 16; k++) peq[k * 64 + lane] = m[k];
-        uint32_t row = scratch[(uint64_t)(j / 32) * stride + tid];
-        uint32_t cout_bits = 0;
-        #pragma unroll
-        for (int k = 0; k < 32; k++) {
-          const uint32_t code = (cw[k >> 2] >> (8 * (k & 3))) & 0xFu;
-          const uint32_t M = peq[code * 64 + lane];
-          uint32_t co;
-          row = lcs_step(row, M, (carrys >> k) & 1u, &co);
-          cout_bits |= co << k;
-        }
-        carrys = cout_bits;
-        scratch[(uint64_t)(j / 32) * stride + tid] = row;
-      }
-    }
-    uint32_t lcs = 0;
-    for (uint32_t w = 0; w < nw; w++) lcs += 32u - __popc(scratch[(uint64_t)w * stride + tid]);
-    jobs[3 * job + 2] = pair_value(A, lcs, R, T);
+    const uint32_t R = A.packed[A.word_off[jobs[3 * job]]];
+    plain_job<false>(A, jobs, job, R, scratch + tid, stride, peq, lane);
   }
 }
 
-static constexpr uint32_t kPlainBlocks = 4096;  // 262,144 threads
-
-uint64_t align_plain_scratch_words(uint32_t nw_max) {
-  return (uint64_t)kPlainBlocks * 64 * (nw_max ? nw_max : 1);
+uint64_t align_plain_scratch_words(uint32_t nw_max, uint32_t max_blocks) {
+  return (uint64_t)max_blocks * 64 * (nw_max ? nw_max : 1);
 }
 
 void launch_align_plain(const AlignArgs& a, uint32_t* jobs, uint32_t count, uint32_t nw_max,
                         uint32_t* scratch, uint64_t scratch_words, hipStream_t stream,
-                        const uint32_t* count_ptr) {
+                        const uint32_t* count_ptr, uint32_t max_blocks) {
   if (count == 0) return;
-  uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)count + 63) / 64, kPlainBlocks);
-  if (blocks > kPlainBlocks) blocks = kPlainBlocks;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)count + 63) / 64, max_blocks);
   const uint32_t stride = blocks * 64;
   if ((uint64_t)stride * nw_max > scratch_words)
     throw HipError{NGIA_ERR_ARG, "plain alignment scratch too small"};

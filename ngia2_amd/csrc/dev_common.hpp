@@ -114,11 +114,16 @@ __host__ __device__ inline uint32_t pair_value(const AlignArgs& A, uint32_t lcs,
 }
 // Plain per-band form: one thread per job triplet, any length.  The rep of
 // every job has at most nw_max words (ceil(R/32)).  With count_ptr set the
-// job count is read from device memory (count is then an upper bound).
+// job count is read from device memory (count is then an upper bound).  The
+// grid has at most max_blocks blocks of 64 threads, which stride over the
+// jobs; the row scratch holds align_plain_scratch_words(nw_max, max_blocks)
+// words.
+constexpr uint32_t kPlainBlocks = 4096;    // 262,144 threads: the plain form
+constexpr uint32_t kPlainBinBlocks = 256;  // the batched form's widest-window bin
 void launch_align_plain(const AlignArgs& a, uint32_t* jobs, uint32_t count, uint32_t nw_max,
                         uint32_t* scratch, uint64_t scratch_words, hipStream_t stream,
-                        const uint32_t* count_ptr = nullptr);
-uint64_t align_plain_scratch_words(uint32_t nw_max);
+                        const uint32_t* count_ptr = nullptr, uint32_t max_blocks = kPlainBlocks);
+uint64_t align_plain_scratch_words(uint32_t nw_max, uint32_t max_blocks = kPlainBlocks);
 // Length-binned forms over a pair list (pairs[list[*off_ptr + i]], i <
 // *count_ptr); work is a zeroed work-queue counter; blocks is the persistent
 // grid.

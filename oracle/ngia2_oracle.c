@@ -845,66 +845,99 @@ static int cmp_u64(const void* a, const void* b) {
   return x < y ? -1 : x > y;
 }
 
+/* Lowest entry of a signature in a per-band array of (sig << 32 | index)
+ * sorted ascending: its index, or 0xFFFFFFFF when the signature is absent. */
+static uint32_t first_of(const uint64_t* row, uint32_t len, uint32_t key) {
+  const uint64_t k = (uint64_t)key << 32;
+  uint32_t lo = 0, hi = len;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (row[mid] < k) lo = mid + 1; else hi = mid;
+  }
+  if (lo == len || (uint32_t)(row[lo] >> 32) != key) return 0xFFFFFFFFu;
+  return (uint32_t)row[lo];
+}
+
 void orc_serve_assign(const uint32_t* packed, const uint64_t* word_off, uint32_t n,
                       const uint32_t* sig, const uint32_t* cluster_of, const uint32_t* qpacked,
                       const uint64_t* qword_off, uint32_t nq, uint32_t identity,
                       uint32_t new_base, uint64_t first_qid, uint32_t* out, int nthreads) {
   if (nthreads <= 0) nthreads = omp_get_max_threads();
   const float thr = (float)identity / 100.0f;
-  /* per band: (sig << 32 | i) sorted, so the first entry of a signature is its lowest i */
-  uint64_t* sorted = (uint64_t*)malloc(sizeof(uint64_t) * ORC_BANDS * (n ? n : 1));
-  for (uint32_t b = 0; b < ORC_BANDS; b++) {
-    uint64_t* row = sorted + (size_t)b * n;
+  /* Query q is snapshot index n + q: it sees the residents and queries 0..q-1. */
+  uint32_t* qsig = (uint32_t*)malloc(sizeof(uint32_t) * ORC_BANDS * (nq ? nq : 1));
+  #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+  for (int64_t q = 0; q < (int64_t)nq; q++)
+    orc_sketch_record(qpacked + qword_off[q], qsig + (size_t)q * ORC_BANDS);
+  /* per band: residents' and queries' (sig << 32 | i) sorted, so the first
+   * entry of a signature is its lowest index (the bucket head rule) */
+  uint64_t* rsorted = (uint64_t*)malloc(sizeof(uint64_t) * ORC_BANDS * (n ? n : 1));
+  uint64_t* qsorted = (uint64_t*)malloc(sizeof(uint64_t) * ORC_BANDS * (nq ? nq : 1));
+  #pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (int b = 0; b < ORC_BANDS; b++) {
+    uint64_t* row = rsorted + (size_t)b * n;
     for (uint32_t i = 0; i < n; i++) row[i] = ((uint64_t)sig[(size_t)i * ORC_BANDS + b] << 32) | i;
     qsort(row, n, sizeof(uint64_t), cmp_u64);
+    uint64_t* qrow = qsorted + (size_t)b * nq;
+    for (uint32_t q = 0; q < nq; q++)
+      qrow[q] = ((uint64_t)qsig[(size_t)q * ORC_BANDS + b] << 32) | q;
+    qsort(qrow, nq, sizeof(uint64_t), cmp_u64);
   }
+  /* best[q] = (score, snapshot index) of q's best candidate; the scores need
+   * no resolution, so every query is scored in parallel */
+  uint32_t* best = (uint32_t*)malloc(sizeof(uint32_t) * 2 * (nq ? nq : 1));
   #pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
   for (int64_t q = 0; q < (int64_t)nq; q++) {
     const uint32_t* qrec = qpacked + qword_off[q];
     const uint32_t Lq = qrec[0];
-    uint32_t qs[ORC_BANDS], heads[ORC_BANDS];
-    orc_sketch_record(qrec, qs);
+    const uint32_t* qs = qsig + (size_t)q * ORC_BANDS;
+    uint32_t heads[ORC_BANDS];
     uint32_t nh = 0;
     for (uint32_t b = 0; b < ORC_BANDS; b++) {
-      const uint64_t* row = sorted + (size_t)b * n;
-      const uint64_t key = (uint64_t)qs[b] << 32;
-      uint32_t lo = 0, hi = n;  /* first entry >= key */
-      while (lo < hi) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (row[mid] < key) lo = mid + 1; else hi = mid;
+      uint32_t h = first_of(rsorted + (size_t)b * n, n, qs[b]);
+      if (h == 0xFFFFFFFFu) {  /* no resident: the first earlier query, if any */
+        const uint32_t p = first_of(qsorted + (size_t)b * nq, nq, qs[b]);
+        if (p >= (uint32_t)q) continue;
+        h = n + p;
       }
-      if (lo == n || (uint32_t)(row[lo] >> 32) != qs[b]) continue;
-      const uint32_t h = (uint32_t)row[lo];
       int seen = 0;
       for (uint32_t k = 0; k < nh; k++) seen |= heads[k] == h;
       if (!seen) heads[nh++] = h;
     }
-    uint32_t best_score = 0, best_res = 0xFFFFFFFFu, cand = 0;
+    uint32_t best_score = 0, best_idx = 0xFFFFFFFFu, cand = 0;
     for (uint32_t k = 0; k < nh; k++) {
       const uint32_t h = heads[k];
-      const uint32_t* hrec = packed + word_off[h];
+      const uint32_t* hrec = h < n ? packed + word_off[h] : qpacked + qword_off[h - n];
       const uint32_t Lr = hrec[0];
-      const uint32_t* rep = Lr >= Lq ? hrec : qrec;
+      const uint32_t* rep = Lr >= Lq ? hrec : qrec;  /* ties: the earlier index */
       const uint32_t* tar = Lr >= Lq ? qrec : hrec;
       if ((float)tar[0] < (float)rep[0] * thr) continue;
       cand++;
       const uint32_t sc = orc_align_record(rep, tar, thr, 1);
-      if (sc > best_score || (sc == best_score && sc > 0 && h < best_res)) {
+      if (sc > best_score || (sc == best_score && sc > 0 && h < best_idx)) {
         best_score = sc;
-        best_res = h;
+        best_idx = h;
       }
     }
+    best[2 * q] = best_score;
+    best[2 * q + 1] = best_idx;
+    out[4 * (size_t)q + 2] = best_score;
+    out[4 * (size_t)q + 3] = cand;
+  }
+  /* clusters in arrival order: a query's best is a resident or an earlier query */
+  for (uint32_t q = 0; q < nq; q++) {
     uint32_t* o = out + 4 * (size_t)q;
-    if (best_score == 0) {
+    const uint32_t h = best[2 * q + 1];
+    if (best[2 * q] == 0) {
       o[0] = new_base + (uint32_t)(first_qid + (uint64_t)q);
       o[1] = 0xFFFFFFFFu;
-      o[2] = 0;
     } else {
-      o[0] = cluster_of[best_res];
-      o[1] = best_res;
-      o[2] = best_score;
+      o[0] = h < n ? cluster_of[h] : out[4 * (size_t)(h - n)];
+      o[1] = h;
     }
-    o[3] = cand;
   }
-  free(sorted);
+  free(best);
+  free(qsorted);
+  free(rsorted);
+  free(qsig);
 }

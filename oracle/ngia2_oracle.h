@@ -98,13 +98,17 @@ int orc_fast_cluster(const uint32_t* packed, const uint64_t* word_off, const uin
                      uint32_t n, const uint32_t* rep_rows, uint32_t bands, uint32_t identity,
                      int fused, int nthreads, uint32_t* cluster, uint64_t* total_pairs);
 /* Serving assignment (no reference counterpart; the rule is stated with
- * ngia_serve_create in include/ngia2.h and built from the pieces above):
- * per query, sketch (makedb.hpp:224-247); per band the lowest resident index
- * with the query's signature (the bucket head of makedb.hpp:340-347); the
- * distinct heads as pairs with the longer sequence as rep (ties: resident),
- * the isJob length test (buildGraph.hpp:262-266) and the kernel_dynamic score;
- * best = highest score, then lowest resident.  out[4q..4q+3] = cluster,
- * resident (0xFFFFFFFF for a new cluster), score, candidates. */
+ * ngia_serve_create in include/ngia2.h and built from the pieces above).
+ * The snapshot grows: query q is snapshot index n + q and sees the residents
+ * and the queries before it.  Per query, sketch (makedb.hpp:224-247); per
+ * band the lowest snapshot index before q with the query's signature (the
+ * bucket head of makedb.hpp:340-347 over residents + earlier queries); the
+ * distinct heads as pairs with the longer sequence as rep (ties: the earlier
+ * index), the isJob length test (buildGraph.hpp:262-266) and the
+ * kernel_dynamic score; best = highest score, then lowest index; a query
+ * joins its best's cluster (a resident's cluster_of, an earlier query's
+ * assigned cluster) or opens new_base + first_qid + q.  out[4q..4q+3] =
+ * cluster, best index (0xFFFFFFFF for a new cluster), score, candidates. */
 void orc_serve_assign(const uint32_t* packed, const uint64_t* word_off, uint32_t n,
                       const uint32_t* sig, const uint32_t* cluster_of, const uint32_t* qpacked,
                       const uint64_t* qword_off, uint32_t nq, uint32_t identity,

@@ -179,6 +179,58 @@ def test_graph_c1(ctx, c1db, identity, mode):
     np.testing.assert_array_equal(got, want)
 
 
+def candidate_union(rep, lengths, bands, identity):
+    """Every band's isJob candidates from the oracle (buildGraph.hpp:262-300):
+    the deduplicated union as sorted rep<<32|tar keys, and the count with
+    duplicates."""
+    per_band = [orc.candidates(rep[b], lengths, identity) for b in range(bands)]
+    allc = np.concatenate(per_band) if per_band else np.zeros(0, np.uint64)
+    return np.unique(allc), len(allc)
+
+
+@pytest.mark.parametrize("identity", [50, 70, 90])
+def test_plain_band_jobs_elementwise(ctx, c1db, identity):
+    """The plain form's job list of every band, element by element and in the
+    reference's order (ascending target), equals the oracle's candidates, and
+    every job's score equals the oracle's."""
+    _, _, d = c1db
+    say("plain band jobs elementwise, identity", identity)
+    db = ng.Database(ctx, d["packed"], d["word_off"], d["lengths"])
+    db.set_reps(d["rep"], 64)
+    thr = float(np.float32(identity) / np.float32(100))
+    seen = 0
+    for b in range(64):
+        jobs = db.band_jobs(identity, b)
+        want = orc.candidates(d["rep"][b], d["lengths"], identity)
+        keys = (jobs[:, 0].astype(np.uint64) << np.uint64(32)) | jobs[:, 1].astype(np.uint64)
+        np.testing.assert_array_equal(keys, want)
+        wj = np.zeros((len(want), 3), dtype=np.uint32)
+        wj[:, 0] = want >> np.uint64(32)
+        wj[:, 1] = want & np.uint64(0xFFFFFFFF)
+        np.testing.assert_array_equal(jobs[:, 2], orc.align_jobs(d["packed"], d["word_off"], wj, thr)[:, 2])
+        seen += len(jobs)
+    db.close()
+    assert seen > 1000
+
+
+@pytest.mark.parametrize("bands", [64, 20])
+@pytest.mark.parametrize("identity", [50, 90])
+def test_batched_candidates_elementwise(ctx, c1db, longdb, identity, bands):
+    """The batched form's distinct pairs are exactly the deduplicated union of
+    the oracle's per-band candidates (no pair dropped, none added), and its
+    counts with and without duplicates are the union's."""
+    for name, (_, _, d) in (("c1", c1db), ("long", longdb)):
+        say("batched candidates elementwise", name, "identity", identity, "bands", bands)
+        db = ng.Database(ctx, d["packed"], d["word_off"], d["lengths"])
+        db.set_reps(np.ascontiguousarray(d["rep"][:bands]), bands)
+        st = db.graph(identity, bands, ng.GRAPH_BATCHED)
+        keys, _ = db.pair_scores()
+        want, tot = candidate_union(d["rep"], d["lengths"], bands, identity)
+        np.testing.assert_array_equal(keys, want)
+        assert st["total_pairs"] == tot and st["distinct_pairs"] == len(want)
+        db.close()
+
+
 @pytest.mark.parametrize("identity", [50, 90])
 def test_batched_pair_scores(ctx, c1db, identity):
     """Every distinct pair's score from the length-binned kernels equals the oracle's."""

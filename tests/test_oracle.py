@@ -1,5 +1,7 @@
 """Pins of the oracle against what the mathematics fixes: closed forms,
 brute force, special cases, an independent textbook LCS.  CPU only."""
+import os
+
 import numpy as np
 import pytest
 
@@ -300,3 +302,85 @@ def test_fast_cluster_brute_force():
     for i in range(n):
         cl.append(i if parent[i] is None else cl[parent[i]])
     np.testing.assert_array_equal(got, np.array(cl, dtype=np.uint32))
+
+
+def graph_from_parts(packed, word_off, lengths, rep, bands, identity):
+    """The edge list of buildGraph.hpp:256-362 assembled in numpy from its
+    pinned pieces: per band the isJob candidates in closed form (buildGraph.hpp:
+    262-266, the loop of test_candidates_closed_form written with numpy), each
+    scored by orc.align_jobs (pinned by the textbook LCS); the passing ones
+    appended as (rep, tar, (float)score / 100.0f) (:339-350); then the
+    lexicographic sort of (u, v, w) and std::unique (:355-362).  Returns the
+    edges and the candidate count with duplicates (:353)."""
+    n = len(lengths)
+    thr = np.float32(identity) / np.float32(100)
+    idx = np.arange(n, dtype=np.uint32)
+    lens = np.asarray(lengths, dtype=np.uint32)
+    u, v, w = [], [], []
+    total = 0
+    for b in range(bands):
+        r = rep[b]
+        ok = (r != idx) & (lens.astype(np.float32) >= lens[r].astype(np.float32) * thr)
+        tar = idx[ok]
+        total += len(tar)
+        if not len(tar):
+            continue
+        jobs = np.zeros((len(tar), 3), dtype=np.uint32)
+        jobs[:, 0] = r[tar]
+        jobs[:, 1] = tar
+        sc = orc.align_jobs(packed, word_off, jobs, float(thr))[:, 2]
+        keep = sc > 0
+        u.append(jobs[keep, 0].astype(np.int32))
+        v.append(jobs[keep, 1].astype(np.int32))
+        w.append(sc[keep].astype(np.float32) / np.float32(100))
+    e = np.zeros(sum(len(x) for x in u), dtype=orc.EDGE_DTYPE)
+    if len(e):
+        e["u"], e["v"], e["w"] = np.concatenate(u), np.concatenate(v), np.concatenate(w)
+        e = e[np.lexsort((e["w"], e["v"], e["u"]))]
+        first = np.ones(len(e), dtype=bool)
+        first[1:] = (e["u"][1:] != e["u"][:-1]) | (e["v"][1:] != e["v"][:-1]) | \
+                    (e["w"][1:] != e["w"][:-1])
+        e = e[first]
+    return e, total
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _golden_db(name):
+    return orc.read_db_numpy(os.path.join(GOLDEN, name))
+
+
+@pytest.mark.parametrize("source", ["edge_cases", "c1_small", "config1_full"])
+def test_build_graph_equals_assembly_from_pinned_parts(tmp_path, source):
+    """orc_build_graph against its parts.  Sets: the reference-built edge-case
+    DB (empty, short, 65,534-residue and unknown-letter records) with its own
+    rows, the reference-built 500-record C1 sample with its rows, and the full
+    10,000-sequence C1 set (200 families of 50) bucketed by the oracle."""
+    import ngia2_amd as ng
+    if source == "config1_full":
+        d = ng.gen_packed(ng.gen_config(1, scale=1.0, seed=1))
+        packed, off, lens = d.packed, d.word_off, d.lengths
+        rep = orc.bucket(orc.sketch_all(packed, off))
+    else:
+        if source == "edge_cases":
+            d = _golden_db("edge_cases.packed")
+        else:  # the oracle's DB of c1_small.fa is byte-equal to the reference's (test_golden)
+            db = str(tmp_path / "c1.packed")
+            assert orc.makedb(os.path.join(GOLDEN, "c1_small.fa"), db) == 0
+            d = orc.read_db_numpy(db)
+        packed, off, lens, rep = d["packed"], d["word_off"], d["lengths"], d["rep"]
+    edges_seen = 0
+    for identity in (30, 50, 70, 90, 97):
+        for bands in (64, 7):
+            want, wtot = graph_from_parts(packed, off, lens, rep, bands, identity)
+            got, tot = orc.build_graph(packed, off, lens, rep, bands, identity)
+            assert tot == wtot
+            np.testing.assert_array_equal(got, want)
+            # weights are exactly score/100 of a score in 1..100, and u < v
+            sc = np.rint(got["w"].astype(np.float64) * 100)
+            assert ((sc >= 1) & (sc <= 100)).all()
+            np.testing.assert_array_equal(sc.astype(np.float32) / np.float32(100), got["w"])
+            assert (got["u"] < got["v"]).all()
+            edges_seen += len(got)
+    assert edges_seen > 0
