@@ -119,6 +119,25 @@ int ngia_align(ngia_ctx* ctx, const uint32_t* packed, uint64_t n_words,
                const uint64_t* word_off, uint32_t n_seqs, uint32_t* jobs,
                uint32_t count, float threshold);
 
+/* (2') The same contract on device memory, allocating nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid hipMalloc).
+ * packed, word_off and jobs are device pointers (the packed area, the uint32
+ * word offset of each record inside it, count job triplets); the scores are
+ * written into jobs in place, asynchronously on `stream` (a hipStream_t; NULL
+ * = the context's stream), and nothing is synchronised.  max_len (at most
+ * NGIA_MAX_LEN, else NGIA_ERR_ARG) is the longest rep among the jobs: it sizes
+ * the per-thread LCS rows; a longer rep is still aligned exactly, on fewer
+ * threads, and a rep of 65,535 residues or more is left unaligned (its score
+ * keeps the caller's value).  The rows live in a static device array of
+ * NGIA_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends and shared
+ * by every call on that GPU: calls that overlap on one GPU (other streams,
+ * contexts or threads) race and must be serialised by the caller. */
+#define NGIA_NOALLOC_SCRATCH_BYTES (32ull << 20)
+int ngia_align_noalloc(ngia_ctx* ctx, const uint32_t* packed, const uint64_t* word_off,
+                       uint32_t* jobs, uint32_t count, float threshold, uint32_t max_len,
+                       void* stream);
+
 /* (3) edge emission: BuildGraph::buildGraph (buildGraph.hpp:256-363).
  * rep_rows: the first `bands` band-major rows; identity 1..99; bands 1..64.
  * *edges receives the sorted unique edge list (release with ngia_free). */

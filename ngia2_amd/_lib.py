@@ -167,6 +167,7 @@ PROTOTYPES = {
     "ngia_sketch": (C.c_int, [P, P, U64, P, U32, P]),
     "ngia_bucket": (C.c_int, [P, P, U32, P]),
     "ngia_align": (C.c_int, [P, P, U64, P, U32, P, U32, C.c_float]),
+    "ngia_align_noalloc": (C.c_int, [P, P, P, P, U32, C.c_float, U32, P]),
     "ngia_build_graph": (C.c_int, [P, P, U64, P, P, U32, P, U32, U32, U32,
                                    C.POINTER(C.POINTER(Edge)), C.POINTER(U64),
                                    C.POINTER(GraphStats)]),

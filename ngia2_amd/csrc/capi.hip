@@ -412,6 +412,7 @@ extern "C" int ngia_open(int device, ngia_ctx** out) {
     NGIA_HIP(hipSetDevice(device));
     NGIA_HIP(hipGetDeviceProperties(&ctx->prop, device));
     ctx->wave_kmax = (uint32_t)align_kernels_setup(ctx->prop.sharedMemPerBlock);
+    ctx->noalloc_scratch = noalloc_setup(device);
     const char* bw = std::getenv("NGIA_BUCKET_WIDE");
     ctx->bucket_wide = bw && bw[0] == '1';
     const char* sb = std::getenv("NGIA_SERIAL_BINS");

@@ -86,6 +86,8 @@ struct ngia_ctx {
   // for a compact alphabet
   bool full_alphabet = false;
   ngia_timing timing{};    // device time of the last sketch / bucket / graph
+  // this device's copy of the row scratch of ngia_align_noalloc (noalloc.hip)
+  uint32_t* noalloc_scratch = nullptr;
   // live handles made on this context; ngia_close detaches them, so a handle
   // destroyed after its context only frees its own memory
   std::vector<ngia_db*> dbs;
@@ -149,6 +151,12 @@ inline void drop_graphs(ngia_db* db) {
 
 // Called by ngia_close for each live serving handle (serve.hip).
 void serve_detach(ngia_serve* sv);
+
+namespace ngia {
+// Device address of the current device's copy of the static row scratch of
+// ngia_align_noalloc (noalloc.hip); looked up once per device.
+uint32_t* noalloc_setup(int device);
+}  // namespace ngia
 
 namespace ngia {
 

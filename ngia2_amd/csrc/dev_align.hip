@@ -219,6 +219,64 @@ __global__ __launch_bounds__(64) void k_align_plain(AlignArgs A, uint32_t* __res
   }
 }
 
+// The same over a scratch of fixed size, in two tiers of threads.  Threads
+// [0, stride_a) take the jobs whose rep has at most nw_cap words, with rows
+// of nw_cap words; threads [stride_a, stride_a + stride_b) take the longer
+// ones, with rows of kPlainMaxWords words (a rep of at most 65,534 residues,
+// the reference's lines[2048]).  Every job is taken by exactly one tier and
+// every row index stays inside the tier's rows, whatever the records hold: a
+// rep longer than kPlainMaxWords words is left unaligned (its score keeps
+// the caller's value, as a length-rejected job's does).
+__global__ __launch_bounds__(64) void k_align_plain_tiers(AlignArgs A, uint32_t* __restrict__ jobs,
+                                                          uint32_t count,
+                                                          uint32_t* __restrict__ scratch,
+                                                          uint32_t nw_cap, uint32_t stride_a,
+                                                          uint32_t stride_b) {
+  __shared__ uint32_t peq[16 * 64];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t tid = blockIdx.x * 64 + lane;
+  const bool tier_a = tid < stride_a;
+  const uint32_t t = tier_a ? tid : tid - stride_a;
+  const uint32_t stride = tier_a ? stride_a : stride_b;
+  uint32_t* rows = tier_a ? scratch + t : scratch + (uint64_t)nw_cap * stride_a + t;
+  for (uint32_t job = t; job < count; job += stride) {
+    const uint32_t R = A.packed[A.word_off[jobs[3 * job]]];
+    const uint32_t nw = (R + 31) / 32;
+    if (tier_a ? nw > nw_cap : (nw <= nw_cap || nw > kPlainMaxWords)) continue;
+    plain_job<true>(A, jobs, job, R, rows, stride, peq, lane);
+  }
+}
+
+uint64_t align_plain_tiers_words(uint32_t nw_cap, uint32_t stride_a, uint32_t stride_b) {
+  return (uint64_t)nw_cap * stride_a + (uint64_t)kPlainMaxWords * stride_b;
+}
+
+void launch_align_plain_tiers(const AlignArgs& a, uint32_t* jobs, uint32_t count,
+                              uint32_t* scratch, uint64_t scratch_words, uint32_t nw_cap,
+                              hipStream_t stream) {
+  if (count == 0) return;
+  if (nw_cap < 1 || nw_cap > kPlainMaxWords)
+    throw HipError{NGIA_ERR_ARG, "row cap outside 1..2048 words"};
+  // Tier b: 1,024 threads with full-length rows (8 MiB); tier a: what the
+  // rest of the scratch holds at nw_cap words per thread, in whole blocks,
+  // at most the plain form's grid and no more than the jobs need.
+  const uint32_t stride_b = 1024;
+  const uint64_t rest = scratch_words - (uint64_t)kPlainMaxWords * stride_b;
+  if (scratch_words < (uint64_t)kPlainMaxWords * stride_b + 64ull * nw_cap)
+    throw HipError{NGIA_ERR_ARG, "scratch too small for two tiers"};
+  uint64_t blocks_a = rest / nw_cap / 64;
+  blocks_a = std::min<uint64_t>(blocks_a, kPlainBlocks);
+  blocks_a = std::min<uint64_t>(blocks_a, ((uint64_t)count + 63) / 64);
+  blocks_a = std::max<uint64_t>(blocks_a, 1);
+  const uint32_t stride_a = (uint32_t)blocks_a * 64;
+  if (align_plain_tiers_words(nw_cap, stride_a, stride_b) > scratch_words)
+    throw HipError{NGIA_ERR_ARG, "plain alignment scratch too small"};
+  const uint32_t blocks = (uint32_t)blocks_a + stride_b / 64;
+  hipLaunchKernelGGL(k_align_plain_tiers, dim3(blocks), dim3(64), 0, stream, a, jobs, count,
+                     scratch, nw_cap, stride_a, stride_b);
+  NGIA_HIP(hipGetLastError());
+}
+
 uint64_t align_plain_scratch_words(uint32_t nw_max, uint32_t max_blocks) {
   return (uint64_t)max_blocks * 64 * (nw_max ? nw_max : 1);
 }

@@ -124,6 +124,14 @@ void launch_align_plain(const AlignArgs& a, uint32_t* jobs, uint32_t count, uint
                         uint32_t* scratch, uint64_t scratch_words, hipStream_t stream,
                         const uint32_t* count_ptr = nullptr, uint32_t max_blocks = kPlainBlocks);
 uint64_t align_plain_scratch_words(uint32_t nw_max, uint32_t max_blocks = kPlainBlocks);
+// The plain kernel over a scratch of fixed size (scratch_words words) and
+// the packed area alone (a.codes unused: target codes come from the planes):
+// jobs whose rep has at most nw_cap words run on threads with nw_cap-word
+// rows, longer ones (up to kPlainMaxWords) on 1,024 threads with full rows.
+constexpr uint32_t kPlainMaxWords = 2048;  // reps of at most 65,534 residues
+void launch_align_plain_tiers(const AlignArgs& a, uint32_t* jobs, uint32_t count,
+                              uint32_t* scratch, uint64_t scratch_words, uint32_t nw_cap,
+                              hipStream_t stream);
 // Length-binned forms over a pair list (pairs[list[*off_ptr + i]], i <
 // *count_ptr); work is a zeroed work-queue counter; blocks is the persistent
 // grid.
