@@ -65,6 +65,8 @@ def kernel_name(b):
         return "k_align_win<%d>" % b["width"]
     if b["kind"] == "wave":
         return "k_align_wave<%d>" % b["width"]
+    if b["kind"] == "seg":
+        return "k_align_seg<%d>" % b["width"]
     return "k_align_plain"
 
 
@@ -147,8 +149,14 @@ def measure(ng, np, ctx, info, d, identity, steps, warmup, plain_steps, label):
     bins = erec[-1]["bins"]
     for i, b in enumerate(bins):
         b["ms"] = float(np.mean([r["bins"][i]["ms"] for r in erec]))
-    out["bins"] = bins
     out["eager_edges_identical"] = bool(np.array_equal(db.edges(), edges))
+    # one more round with the tally on: the reference tiles of the blocks each
+    # bin's kernel ran (a pair that can no longer pass ends early), untimed
+    db.graph(identity, 64, ng.GRAPH_BATCHED_TALLY)
+    for b, t in zip(bins, db.bin_stats()):
+        b["run_tiles"] = t["run_tiles"]
+    out["tally_edges_identical"] = bool(np.array_equal(db.edges(), edges))
+    out["bins"] = bins
     out["edges"] = edges
 
     if plain_steps:
@@ -286,20 +294,31 @@ def serving_bench(ng, np, ctx, d, params, identity, n_queries, seed):
 
 
 def align_rooflines(m, peak):
+    """Per alignment kernel and for the whole phase: `achieved` counts the
+    reference tiles of the blocks the kernels ran (run_tiles, from the tally
+    round; an early exit skips the rest of a failing pair), so `frac` is the
+    share of the VALU peak doing that work.  `ref_rate` counts every
+    reference tile of the pairs, the work the reference's loop does for them:
+    with early exits it can exceed the peak, as it measures skipped work too."""
     rows = []
     for b in m["bins"]:
-        if b["kind"] not in ("window", "wave", "plain") or not b["pairs"] or b["ms"] <= 0:
+        if b["kind"] not in ("window", "wave", "seg", "plain") or not b["pairs"] or b["ms"] <= 0:
             continue
-        ops = b["ref_tiles"] * OPS_PER_TILE
-        ach = ops / (b["ms"] * 1e-3) / 1e12
+        sec = b["ms"] * 1e-3
+        ach = b["run_tiles"] * OPS_PER_TILE / sec / 1e12
         rows.append({"kernel": kernel_name(b), "pairs": b["pairs"], "ref_tiles": b["ref_tiles"],
-                     "kernel_ms": round(b["ms"], 4), "achieved": round(ach, 3),
-                     "frac": round(ach / peak, 4)})
+                     "run_tiles": b["run_tiles"], "kernel_ms": round(b["ms"], 4),
+                     "achieved": round(ach, 3), "frac": round(ach / peak, 4),
+                     "ref_rate": round(b["ref_tiles"] * OPS_PER_TILE / sec / 1e12, 3)})
     rows.sort(key=lambda r: -r["kernel_ms"])
     tiles = sum(b["ref_tiles"] for b in m["bins"])
-    phase = tiles * OPS_PER_TILE / (m["align_ms"] * 1e-3) / 1e12 if m["align_ms"] > 0 else 0.0
-    return rows, {"ref_tiles": tiles, "ms": round(m["align_ms"], 4), "achieved": round(phase, 3),
-                  "frac": round(phase / peak, 4)}
+    run = sum(b.get("run_tiles", 0) for b in m["bins"])
+    sec = m["align_ms"] * 1e-3
+    phase = run * OPS_PER_TILE / sec / 1e12 if sec > 0 else 0.0
+    ref_rate = tiles * OPS_PER_TILE / sec / 1e12 if sec > 0 else 0.0
+    return rows, {"ref_tiles": tiles, "run_tiles": run, "ms": round(m["align_ms"], 4),
+                  "achieved": round(phase, 3), "frac": round(phase / peak, 4),
+                  "ref_rate": round(ref_rate, 3)}
 
 
 def chunked_graph(ng, np, ctx, d, identity, blocks):
@@ -465,12 +484,24 @@ def main():
     if not args.no_tune:
         t0 = time.perf_counter()
         pts = ctx.autotune(args.identity)
-        route = ctx.tuning()
+        t = ctx.tuning()
+        sizes = [c for c in t["class_size"] if c]
+
+        def shapes(table):
+            # per window width: the shape chosen for each bin-size class
+            return {str(W): {str(sizes[k]): ng.SHAPES[table[W][k]] for k in range(len(sizes))}
+                    for W in range(2, 17)}
         tune = {"identity": args.identity, "seconds": round(time.perf_counter() - t0, 3),
-                "route_below": {str(W): route[W] for W in range(2, 17) if route[W]},
-                "points": [[p["width"], p["pairs"], round(p["window_ms"], 4), round(p["wave_ms"], 4)]
-                           for p in pts]}
-        log("autotune: %.2f s, route_below %s" % (tune["seconds"], tune["route_below"]))
+                "class_size": sizes, "shapes": shapes(t["shape"]),
+                "shapes_compact": shapes(t["shape_compact"]),
+                "points": [[p["width"], p["dna"], p["pairs"]] +
+                           [round(p["ms"].get(x, 0.0), 4) for x in ng.SHAPES] for p in pts],
+                "points_columns": ["width", "dna", "pairs"] + ["%s_ms" % x for x in ng.SHAPES],
+                "fixed": "K per wavefront bin is the smallest that holds the band window "
+                         "(64K >= W + K - 1); windows of 17-31 words take 32 lanes per pair; "
+                         "persistent grids are as many blocks as the CUs hold (LDS-bound)"}
+        log("autotune: %.2f s, shapes %s" % (tune["seconds"], {
+            W: sorted(set(v.values())) for W, v in tune["shapes"].items()}))
     traffic = load_traffic()
 
     t0 = time.perf_counter()
@@ -509,8 +540,9 @@ def main():
                     "peak": round(peak, 3), "unit": "TOP/s int32", "frac": a["frac"],
                     "traffic": traffic.get(a["kernel"], {}).get(workload),
                     "traffic_source": TRAFFIC_SOURCE,
-                    "kernel_ms": a["kernel_ms"], "algorithmic_ops": a["ref_tiles"] * OPS_PER_TILE,
-                    "ref_tiles": a["ref_tiles"], "pairs": a["pairs"]}
+                    "kernel_ms": a["kernel_ms"], "algorithmic_ops": a["run_tiles"] * OPS_PER_TILE,
+                    "ref_tiles": a["ref_tiles"], "run_tiles": a["run_tiles"],
+                    "ref_rate": a["ref_rate"], "pairs": a["pairs"]}
 
     out = {"metric": "seqs clustered/s + aligned pairs/s at -i %d (1 GPU); p50/p99 stream latency"
                      % args.identity,
@@ -548,7 +580,9 @@ def main():
            "roofline": roofline,
            "sketch_roofline": sketch_roof,
            "align_phase": dict(phase, peak=round(peak, 3), unit="TOP/s int32",
-                               eager_ms_per_step=round(m["eager_ms_per_step"], 3)),
+                               eager_ms_per_step=round(m["eager_ms_per_step"], 3),
+                               eager_edges_identical=m["eager_edges_identical"],
+                               tally_edges_identical=m["tally_edges_identical"]),
            "align_kernels": align_rows[:8],
            "autotune": tune,
            "device": {"name": info["name"], "arch": info["arch"], "cu": info["cu_count"],
@@ -600,7 +634,8 @@ def main():
                  "aligned_pairs_per_s": round(m2["stats"]["total_pairs"] /
                                               ((m2["graph_ms"] + m2["finalize_ms"]) * 1e-3), 1),
                  "align_phase": phase2, "align_kernels": rows2[:4],
-                 "eager_edges_identical": m2["eager_edges_identical"]}
+                 "eager_edges_identical": m2["eager_edges_identical"],
+                 "tally_edges_identical": m2["tally_edges_identical"]}
             ops2, nbytes2 = sketch_work(d2.lengths)
             r["sketch_hbm_gbps"] = round(nbytes2 / (m2["sketch_ms"] * 1e-3) / 1e9, 2)
             if "plain" in m2:

@@ -52,6 +52,9 @@ typedef struct {
 #define NGIA_GRAPH_BATCHED 1u  /* cross-band dedupe, length-binned kernels, captured hipGraphs */
 #define NGIA_GRAPH_BATCHED_EAGER 2u /* the batched kernels launched without the graph, with
                                      * the alignment phase timed (ngia_timing.align_ms) */
+#define NGIA_GRAPH_BATCHED_TALLY 3u /* as EAGER, and each alignment kernel also counts the
+                                     * reference tiles of the blocks it ran (ngia_bin_stat.
+                                     * run_tiles); the counting slows that call */
 
 typedef struct {
   uint64_t total_pairs;     /* candidates summed over bands, duplicates included ("compare pairs", buildGraph.hpp:353) */
@@ -209,49 +212,67 @@ int ngia_db_band_jobs(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ba
 /* Length bins of the last batched graph.  No reference counterpart (the
  * reference aligns every band's jobs in one kernel_dynamic launch).  A pair's
  * bin follows from its band window W = lsft/32 + ceil(rsft/32) words:
- * W <= 16 -> the thread-per-pair window kernel of width W; W <= 64*K -> the
- * wavefront-per-pair kernel with K words per lane; wider -> the plain kernel;
- * length-rejected or empty targets score 0 without a kernel. */
+ * W <= 16 -> a window bin, aligned by the shape the tuning picks for its size
+ * (one thread per pair by default); W <= 31 -> 32 lanes per pair, two pairs
+ * per wavefront; W <= 64*K - K + 1 -> a wavefront per pair with K words per
+ * lane; wider -> the plain kernel; length-rejected or empty targets score 0
+ * without a kernel. */
 #define NGIA_BIN_ZERO 0u
-#define NGIA_BIN_WINDOW 1u
-#define NGIA_BIN_WAVE 2u
+#define NGIA_BIN_WINDOW 1u   /* k_align_win<W>: one thread per pair; width = W */
+#define NGIA_BIN_WAVE 2u     /* k_align_wave<K>: one wavefront per pair; width = K */
 #define NGIA_BIN_PLAIN 3u
+#define NGIA_BIN_SEG 4u      /* k_align_seg<G>: 64/G pairs per wavefront; width = G */
 typedef struct {
   uint32_t kind;       /* NGIA_BIN_* */
-  uint32_t width;      /* W (window) or K (wave); 0 otherwise */
+  uint32_t width;      /* W (window), K (wave), G (seg); 0 otherwise */
   uint64_t pairs;      /* distinct pairs in the bin */
   uint64_t ref_tiles;  /* reference 32x32 tiles of those pairs */
   double ms;           /* the bin's kernel time on its stream (NGIA_GRAPH_BATCHED_EAGER), else 0 */
+  uint64_t run_tiles;  /* NGIA_GRAPH_BATCHED_TALLY: the reference tiles of the column blocks
+                        * the kernel ran (a pair that can no longer pass ends early), else 0 */
 } ngia_bin_stat;
 int ngia_db_bin_stats(ngia_ctx* ctx, ngia_db* db, ngia_bin_stat* out, uint32_t cap,
                       uint32_t* n_bins);
 
 /* ------------------------------------------- autotuner */
-/* No reference counterpart.  The batched form aligns a window bin (band
- * window W = 2..16 words) with one thread per pair; a bin with few pairs
- * finishes sooner with one wavefront per pair (k_align_wave<1>), because a
- * lone thread runs its pair's T x W word-characters serially.  Where the two
- * cross depends on W, the identity (pair length) and the device. */
+/* No reference counterpart; the reference has one fixed shape, a thread per
+ * job in blocks of 64 (buildGraph.hpp:192, 323).  A window bin (band window
+ * W = 2..16 words) can be aligned in four shapes: one thread per pair (the
+ * most work per instruction, but a lone pair's T x W word-characters run
+ * serially on one lane), 16 or 32 lanes per pair (several pairs per
+ * wavefront, one word per lane) or a wavefront per pair.  Which is fastest
+ * depends on W, the bin's size, the identity (pair length) and the device. */
+#define NGIA_SHAPE_WINDOW 0u  /* k_align_win<W> */
+#define NGIA_SHAPE_SEG16 1u   /* k_align_seg<16>: four pairs per wavefront (W <= 15) */
+#define NGIA_SHAPE_SEG32 2u   /* k_align_seg<32>: two pairs per wavefront */
+#define NGIA_SHAPE_WAVE 3u    /* k_align_wave<1> */
+#define NGIA_TUNE_CLASSES 8
 typedef struct {
-  /* window bin W goes to the wavefront kernel when it holds fewer than
-   * route_below[W] pairs; 0 = never (the state of a fresh context) */
-  uint32_t route_below[17];
+  /* window bin W holding c pairs takes shape[W][k], k the last size class with
+   * class_size[k] <= c (class 0 below class_size[0]); class sizes ascend, 0
+   * ends the list.  A fresh context's table is all zero: every window bin on
+   * one thread per pair. */
+  uint32_t class_size[NGIA_TUNE_CLASSES];
+  uint32_t shape[17][NGIA_TUNE_CLASSES];
+  /* the same for databases whose residues use at most five codes (DNA), whose
+   * window kernels index a compact table */
+  uint32_t shape_compact[17][NGIA_TUNE_CLASSES];
 } ngia_tuning;
 int ngia_get_tuning(ngia_ctx* ctx, ngia_tuning* out);
+/* NGIA_ERR_ARG for an unknown shape or NGIA_SHAPE_SEG16 at W = 16. */
 int ngia_set_tuning(ngia_ctx* ctx, const ngia_tuning* t);
 typedef struct {
   uint32_t width;    /* band window W */
+  uint32_t dna;      /* 1: the DNA pass (compact table), 0: proteins */
   uint32_t length;   /* rep length of the synthetic pairs (targets are up to 64 shorter) */
   uint32_t pairs;    /* pairs in the timed bin */
-  double window_ms;  /* k_align_win<W>, device time (HIP events) */
-  double wave_ms;    /* k_align_wave<1> on the same pairs */
+  double ms[4];      /* device time (HIP events) per NGIA_SHAPE_*, 0 where not applicable */
 } ngia_tune_point;
-/* Time both kernels on synthetic pairs of every window width W at `identity`
+/* Time every shape on synthetic pairs of every window width W at `identity`
  * (random reps with targets of nearly the same length, chosen so that the
- * band window is exactly W),
- * at bins of 16..4096 pairs, and install the crossovers in the context's
- * tuning: route_below[W] = the smallest timed bin size at which the window
- * kernel is no slower.  points (may be NULL) receive the measurements. */
+ * band window is exactly W; proteins, then DNA), at bins of 16..65536 pairs,
+ * and install the fastest shape per (W, size class) in the context's tuning.
+ * points (may be NULL) receive the measurements. */
 int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* points, uint32_t cap,
                   uint32_t* n_points);
 

@@ -11,8 +11,9 @@ import weakref
 import numpy as np
 
 from . import _lib
-from ._lib import (BANDS, GRAPH_BATCHED, GRAPH_BATCHED_EAGER, GRAPH_PLAIN, NGIA_ERR_ARG, NGIA_ERR_IO,
-                   NGIA_ERR_NODEV, NGIA_ERR_STATE, NGIA_OK, BIN_DIR, LIB_PATH)
+from ._lib import (BANDS, GRAPH_BATCHED, GRAPH_BATCHED_EAGER, GRAPH_BATCHED_TALLY, GRAPH_PLAIN,
+                   NGIA_ERR_ARG, NGIA_ERR_IO, NGIA_ERR_NODEV, NGIA_ERR_STATE, NGIA_OK, BIN_DIR,
+                   LIB_PATH, SHAPE_WINDOW, SHAPE_SEG16, SHAPE_SEG32, SHAPE_WAVE, SHAPES)
 
 __all__ = ["NgiaError", "Context", "Database", "PackedDB", "Serve", "Queries", "gen_config",
            "gen_fasta", "gen_packed", "gen_queries", "read_db", "leiden", "resolution",
@@ -79,26 +80,44 @@ class Context:
             self.h = None
 
     def tuning(self):
-        """route_below[W] of the window bins (ngia_get_tuning)."""
+        """Shapes of the window bins by size (ngia_get_tuning): a dict with
+        class_size (8 ascending bin sizes, 0 ends the list) and shape /
+        shape_compact, 17 x 8 lists of SHAPE_* for window width W and size
+        class k (the 16-code table and compact alphabets)."""
         t = _lib.Tuning()
         _check(_lib.lib().ngia_get_tuning(self.h, C.byref(t)), self.h)
-        return list(t.route_below)
+        return {"class_size": list(t.class_size),
+                "shape": [list(t.shape[W]) for W in range(17)],
+                "shape_compact": [list(t.shape_compact[W]) for W in range(17)]}
 
-    def set_tuning(self, route_below):
+    def set_tuning(self, tuning):
         t = _lib.Tuning()
-        for W, v in enumerate(route_below):
-            t.route_below[W] = int(v)
+        for k, v in enumerate(tuning.get("class_size", [])):
+            t.class_size[k] = int(v)
+        for name in ("shape", "shape_compact"):
+            for W, row in enumerate(tuning.get(name, [])):
+                for k, v in enumerate(row):
+                    getattr(t, name)[W][k] = int(v)
         _check(_lib.lib().ngia_set_tuning(self.h, C.byref(t)), self.h)
 
+    def route_all(self, shape):
+        """Send every window bin, whatever its size, to one shape (SHAPE_SEG16
+        falls back to 32 lanes at W = 16); SHAPE_WINDOW restores the default."""
+        row = lambda W: [_lib.SHAPE_SEG32 if (shape == _lib.SHAPE_SEG16 and W == 16) else shape] * 8
+        self.set_tuning({"class_size": [1, 0, 0, 0, 0, 0, 0, 0],
+                         "shape": [row(W) for W in range(17)],
+                         "shape_compact": [row(W) for W in range(17)]})
+
     def autotune(self, identity):
-        """Time the window and wavefront kernels per window width and install the
-        crossovers (ngia_autotune); returns the measured points."""
-        cap = 256
+        """Time every window-bin shape per window width and bin size, proteins
+        then DNA, and install the fastest (ngia_autotune); returns the points."""
+        cap = 512
         pts = (_lib.TunePoint * cap)()
         n = C.c_uint32()
         _check(_lib.lib().ngia_autotune(self.h, identity, pts, cap, C.byref(n)), self.h)
-        return [{"width": p.width, "length": p.length, "pairs": p.pairs,
-                 "window_ms": p.window_ms, "wave_ms": p.wave_ms} for p in pts[:min(n.value, cap)]]
+        return [{"width": p.width, "dna": p.dna, "length": p.length, "pairs": p.pairs,
+                 "ms": {_lib.SHAPES[k]: p.ms[k] for k in range(4) if p.ms[k] > 0}}
+                for p in pts[:min(n.value, cap)]]
 
     def __enter__(self):
         return self
@@ -284,14 +303,16 @@ class Database:
         return jobs[:c.value].copy()
 
     def bin_stats(self):
-        """Length bins of the last batched graph: kind, width, pairs, ref_tiles, ms."""
+        """Length bins of the last batched graph: kind, width, pairs, ref_tiles, ms and
+        run_tiles (the reference tiles of the blocks the kernels ran; counted by
+        GRAPH_BATCHED_TALLY rounds only, else 0)."""
         L = _lib.lib()
         nb = C.c_uint32()
         _check(L.ngia_db_bin_stats(self.ctx.h, self.h, None, 0, C.byref(nb)), self.ctx.h)
         arr = (_lib.BinStat * nb.value)()
         _check(L.ngia_db_bin_stats(self.ctx.h, self.h, arr, nb.value, C.byref(nb)), self.ctx.h)
         return [{"kind": _lib.BIN_KINDS[b.kind], "width": b.width, "pairs": b.pairs,
-                 "ref_tiles": b.ref_tiles, "ms": b.ms} for b in arr]
+                 "ref_tiles": b.ref_tiles, "ms": b.ms, "run_tiles": b.run_tiles} for b in arr]
 
     def fast(self, identity, bands=BANDS):
         cl = np.zeros(self.n, dtype=np.uint32)

@@ -20,6 +20,7 @@ NGIA_ERR_NODEV = -6
 GRAPH_PLAIN = 0
 GRAPH_BATCHED = 1
 GRAPH_BATCHED_EAGER = 2
+GRAPH_BATCHED_TALLY = 3
 BANDS = 64
 
 
@@ -47,19 +48,25 @@ class GraphStats(C.Structure):
 
 class BinStat(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("width", C.c_uint32), ("pairs", C.c_uint64),
-                ("ref_tiles", C.c_uint64), ("ms", C.c_double)]
+                ("ref_tiles", C.c_uint64), ("ms", C.c_double), ("run_tiles", C.c_uint64)]
 
 
-BIN_KINDS = {0: "zero", 1: "window", 2: "wave", 3: "plain"}
+BIN_KINDS = {0: "zero", 1: "window", 2: "wave", 3: "plain", 4: "seg"}
+
+SHAPE_WINDOW, SHAPE_SEG16, SHAPE_SEG32, SHAPE_WAVE = 0, 1, 2, 3
+SHAPES = ("window", "seg16", "seg32", "wave")
+TUNE_CLASSES = 8
 
 
 class Tuning(C.Structure):
-    _fields_ = [("route_below", C.c_uint32 * 17)]
+    _fields_ = [("class_size", C.c_uint32 * TUNE_CLASSES),
+                ("shape", (C.c_uint32 * TUNE_CLASSES) * 17),
+                ("shape_compact", (C.c_uint32 * TUNE_CLASSES) * 17)]
 
 
 class TunePoint(C.Structure):
-    _fields_ = [("width", C.c_uint32), ("length", C.c_uint32), ("pairs", C.c_uint32),
-                ("window_ms", C.c_double), ("wave_ms", C.c_double)]
+    _fields_ = [("width", C.c_uint32), ("dna", C.c_uint32), ("length", C.c_uint32),
+                ("pairs", C.c_uint32), ("ms", C.c_double * 4)]
 
 
 class Timing(C.Structure):

@@ -163,21 +163,28 @@ void enqueue_bins(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands)
   const uint32_t grid = persistent_blocks(ctx, 0) * 4;
   bin_pairs(db->pairs.as<uint2>(), npairs, len, thr, db->bin_of.as<uint32_t>(), dc->bin_cnt,
             dc->bin_off, dc->cursor, db->list.as<uint32_t>(), dc->bin_tiles, db->score.as<uint32_t>(),
-            ctx->wave_kmax, dc->remap, ctx->route, grid, s);
+            ctx->wave_kmax, dc->remap, db->compact ? ctx->route_compact : ctx->route, grid, s);
 }
 
 // Part 2: the alignment kernels of the bins in `mask` (bit b: bin b holds
 // pairs), then the passing pairs as raw edges (capturable).
 void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
-                   bool time_align) {
+                   bool time_align, bool tally = false) {
   hipStream_t s = ctx->stream;
   const uint32_t n = db->n;
   const float thr = (float)identity / 100.0f;
   Counters* dc = db->counters.as<Counters>();
   const uint64_t* npairs = db->off.as<uint64_t>() + n;
   const uint32_t grid = persistent_blocks(ctx, 0) * 4;
-  const AlignArgs aa = align_args(db, thr);
+  const AlignArgs a0 = align_args(db, thr);
   auto holds = [&](int bin) { return ((mask >> bin) & 1u) != 0; };
+  // The arguments of bin b's kernel: with the tally on, it counts the
+  // reference tiles of the blocks it ran into run_tiles[b].
+  auto args = [&](int bin) {
+    AlignArgs a = a0;
+    if (tally) a.tally = &dc->run_tiles[bin];
+    return a;
+  };
   // The bins run concurrently, forked onto the side streams and joined back.
   // Every wave bin below K = 16 is latency bound (one pair per wavefront, the
   // bin lasts as long as its longest pair), so each gets a side stream of its
@@ -197,7 +204,7 @@ void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
     const int bin = kBinWave0 + k;
     if ((1u << k) > ctx->wave_kmax || !holds(bin)) return;
     mark(bin, 0, q);
-    launch_align_wave(1 << k, aa, db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[bin],
+    launch_align_wave(1 << k, args(bin), db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[bin],
                       &dc->bin_cnt[bin], &dc->work[bin], db->score.as<uint32_t>(),
                       persistent_blocks(ctx, wave_lds_bytes(1 << k)), q);
     mark(bin, 1, q);
@@ -209,15 +216,26 @@ void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
     for (int k = 0; k < kSideStreams; k++) NGIA_HIP(hipStreamWaitEvent(ctx->side[k], ctx->fork, 0));
   }
   for (int k = 3; k >= 0; k--) wave(k, fork ? ctx->side[3 - k] : s);
+  // The segmented bins (several pairs per wavefront) follow K = 8 and K = 1.
+  auto seg = [&](int bin, hipStream_t q) {
+    if (!holds(bin)) return;
+    mark(bin, 0, q);
+    launch_align_seg(bin == kBinSeg16 ? 16 : 32, args(bin), db->pairs.as<uint2>(), db->list.as<uint32_t>(),
+                     &dc->bin_off[bin], &dc->bin_cnt[bin], &dc->work[bin], db->score.as<uint32_t>(),
+                     persistent_blocks(ctx, seg_lds_bytes()), q);
+    mark(bin, 1, q);
+  };
+  seg(kBinSeg32, fork ? ctx->side[0] : s);
+  seg(kBinSeg16, fork ? ctx->side[3] : s);
   static constexpr int kWinStreams[3] = {0, 3, 2};
   int slot = 0;
   for (int W = kWinMax; W >= 2; W--) {
     if (!holds(W)) continue;
     hipStream_t q = fork ? ctx->side[kWinStreams[slot++ % 3]] : s;
     mark(W, 0, q);
-    launch_align_win(W, aa, db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[W],
+    launch_align_win(W, args(W), db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[W],
                      &dc->bin_cnt[W], &dc->work[W], db->score.as<uint32_t>(),
-                     persistent_blocks(ctx, win_lds_bytes(W, aa.slots != nullptr)), q);
+                     persistent_blocks(ctx, win_lds_bytes(W, a0.slots != nullptr)), q);
     mark(W, 1, q);
   }
   for (int k = 0; fork && k < kSideStreams; k++) {
@@ -231,7 +249,7 @@ void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
                    db->plain_jobs.as<uint32_t>(), s);
     const uint32_t nw_max = (db->max_len + 31) / 32;
     mark(kBinPlain, 0, s);
-    launch_align_plain(aa, db->plain_jobs.as<uint32_t>(), 0xFFFFFFFFu, nw_max,
+    launch_align_plain(args(kBinPlain), db->plain_jobs.as<uint32_t>(), 0xFFFFFFFFu, nw_max,
                        db->plain_scratch.as<uint32_t>(), db->plain_scratch.bytes / 4, s,
                        &dc->bin_cnt[kBinPlain], kPlainBinBlocks);
     mark(kBinPlain, 1, s);
@@ -296,7 +314,7 @@ bool ensure_plain_bin_space(ngia_db* db, uint32_t bands) {
 // second once per set of bins that hold pairs, both replayed (or launched
 // eagerly, with the alignment phase timed).
 void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
-                   ngia_graph_stats* st, bool eager) {
+                   ngia_graph_stats* st, bool eager, bool tally) {
   hipStream_t s = ctx->stream;
   const uint32_t n = db->n;
   ensure_batched_space(db, bands);
@@ -307,7 +325,12 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
                           &db->rep, &db->counters, &db->cnt, &db->keep, &db->off, &db->sums, &db->pairs,
                           &db->score, &db->bin_of, &db->list, &db->ekeys, &db->escores})
     key.push_back((uintptr_t)b->p);
-  for (uint32_t w : ctx->route.route_below) key.push_back(w);
+  {
+    const ngia::RouteTable& rt = db->compact ? ctx->route_compact : ctx->route;
+    for (uint32_t w : rt.class_size) key.push_back(w);
+    for (int W = 0; W <= kWinMax; W++)
+      for (int k = 0; k < kTuneClasses; k++) key.push_back(rt.shape[W][k]);
+  }
   const bool reuse = db->gexec && db->g_identity == identity && db->g_bands == bands &&
                      db->g_key == key;
   if (!eager && !reuse) {
@@ -342,7 +365,7 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   }
   if (eager) {
     // Same kernels without the graph, with events around the alignment phase.
-    enqueue_align(ctx, db, identity, mask, /*time_align=*/true);
+    enqueue_align(ctx, db, identity, mask, /*time_align=*/true, tally);
   } else {
     hipGraphExec_t g2 = nullptr;
     for (const auto& e : db->gexec2)
@@ -375,7 +398,8 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   db->have_bins = true;
   uint32_t launches = 0;
   for (int b = 0; b < kBins; b++) {
-    const bool aligner = b == kBinPlain || (b >= 2 && b <= kWinMax) ||
+    const bool aligner = b == kBinPlain || b == kBinSeg16 || b == kBinSeg32 ||
+                         (b >= 2 && b <= kWinMax) ||
                          (b >= kBinWave0 && b < kBinPlain && (1u << (b - kBinWave0)) <= ctx->wave_kmax);
     const bool launched = aligner && ((mask >> b) & 1u);
     launches += launched;
@@ -587,8 +611,10 @@ extern "C" int ngia_db_graph(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint
   if (st) std::memset(st, 0, sizeof(*st));
   return guarded(ctx, [&] {
     auto t0 = std::chrono::steady_clock::now();
-    if (flags == NGIA_GRAPH_BATCHED || flags == NGIA_GRAPH_BATCHED_EAGER)
-      graph_batched(ctx, db, identity, bands, st, flags == NGIA_GRAPH_BATCHED_EAGER);
+    if (flags == NGIA_GRAPH_BATCHED || flags == NGIA_GRAPH_BATCHED_EAGER ||
+        flags == NGIA_GRAPH_BATCHED_TALLY)
+      graph_batched(ctx, db, identity, bands, st, flags != NGIA_GRAPH_BATCHED,
+                    flags == NGIA_GRAPH_BATCHED_TALLY);
     else
       graph_plain(ctx, db, identity, bands, st, nullptr);
     auto t1 = std::chrono::steady_clock::now();
@@ -683,13 +709,17 @@ extern "C" int ngia_db_bin_stats(ngia_ctx* ctx, ngia_db* db, ngia_bin_stat* out,
     } else if (b < kBinPlain) {
       o.kind = NGIA_BIN_WAVE;
       o.width = 1u << (b - kBinWave0);
-    } else {
+    } else if (b == kBinPlain) {
       o.kind = NGIA_BIN_PLAIN;
       o.width = 0;
+    } else {
+      o.kind = NGIA_BIN_SEG;
+      o.width = b == kBinSeg16 ? 16u : 32u;
     }
     o.pairs = db->last.bin_cnt[b];
     o.ref_tiles = db->last.bin_tiles[b];
     o.ms = db->bin_ms[b];
+    o.run_tiles = db->last.run_tiles[b];
   }
   return NGIA_OK;
 }

@@ -54,6 +54,7 @@ struct Counters {
   unsigned long long ecount;      // raw edges
   unsigned long long unique;      // edges after unique
   unsigned long long bin_tiles[kBins];  // reference tiles of the pairs in each bin
+  unsigned long long run_tiles[kBins];  // tally: reference tiles of the blocks run
   uint32_t bin_cnt[kBins];
   uint32_t bin_off[kBins + 1];
   uint32_t cursor[kBins];
@@ -75,7 +76,9 @@ struct ngia_ctx {
   // start/end of each bin's kernel in the eager batched form
   hipEvent_t bin_ev[2 * ngia::kBins] = {};
   uint32_t wave_kmax = 8;  // largest k_align_wave<K> usable on this device
-  ngia::RouteTable route{};  // small window bins to the wavefront kernel (autotuner)
+  // shapes of the window bins by size (autotuner): the 16-code table, and
+  // databases of a compact alphabet
+  ngia::RouteTable route{}, route_compact{};
   // NGIA_BUCKET_WIDE=1 selects the 38-bit-key bucketing that n >= 2^26 uses
   bool bucket_wide = false;
   // NGIA_SERIAL_BINS=1 runs the alignment bins one after another on the

@@ -213,10 +213,16 @@ __global__ __launch_bounds__(64) void k_align_plain(AlignArgs A, uint32_t* __res
   const uint32_t lane = threadIdx.x;
   const uint32_t tid = blockIdx.x * 64 + lane;
   if (count_ptr) count = *count_ptr;
+  uint64_t ran = 0;  // tally: reference tiles of the jobs run
   for (uint32_t job = tid; job < count; job += stride) {
     const uint32_t R = A.packed[A.word_off[jobs[3 * job]]];
     plain_job<false>(A, jobs, job, R, scratch + tid, stride, peq, lane);
+    if (A.tally) {
+      const uint32_t T = A.packed[A.word_off[jobs[3 * job + 1]]];
+      if (!length_rejects(R, T, A.thr)) ran += ref_tiles(R, T, A.thr, (T + 31) / 32);
+    }
   }
+  tally_add(A, ran);
 }
 
 // The same over a scratch of fixed size, in two tiers of threads.  Threads
@@ -294,6 +300,18 @@ void launch_align_plain(const AlignArgs& a, uint32_t* jobs, uint32_t count, uint
   NGIA_HIP(hipGetLastError());
 }
 
+// Early exit.  The kernels compute the LCS of the band-masked match matrix
+// (the bit-parallel update with the masks of words outside the band set to
+// 0 is exactly the banded recurrence), and an LCS grows by at most one per
+// target character; the pass test (buildGraph.hpp:248) is monotone in the
+// LCS.  So a pair whose LCS after `done` characters, plus the characters left,
+// fails the test scores 0 whatever follows.
+__device__ inline bool cannot_pass(uint32_t lcs_now, uint32_t done, uint32_t R, uint32_t T,
+                                   float thr) {
+  const uint32_t left = T > done ? T - done : 0u;
+  return lcs_score(lcs_now + left, R, T, thr) == 0u;
+}
+
 // ------------------------------------------------ short window, thread/pair
 // Pair list entry: pairs[p] = (rep, tar); geometry computed by the binning
 // pass so that a + c == W for every pair of the list.
@@ -309,7 +327,7 @@ struct WinPair {
   const uint32_t* tc;    // target byte codes, 8 dwords per block
   int32_t c, nw;
   uint32_t nb;
-  uint32_t* lds;         // [W][NC][64] Peq slots, then [8][64] staged codes
+  uint32_t* lds;         // [W][NC][64] Peq slots
   uint32_t lane, lane4;
   uint4 nc0, nc1;        // codes of the next block, loaded one block ahead
   uint32_t slot_code;    // NC < 16: the code of slot s (1..NC-1) in bits 4(s-1)..
@@ -337,18 +355,55 @@ __device__ __forceinline__ void win_store_masks(uint32_t* slot, uint32_t lane, u
   }
 }
 
+// The masks of one group of four characters (code bytes of cw) for every
+// window register: M[mm][r] = Peq slot r's mask of character mm's code.
+template <int W, int NC>
+__device__ __forceinline__ void win_load_group(const uint32_t* lds, uint32_t cw, uint32_t lane4,
+                                               uint32_t (&M)[4][W]) {
+  constexpr int kSlot = NC * 64;
+  #pragma unroll
+  for (int mm = 0; mm < 4; mm++) {
+    // byte address of Peq[code][lane] = code*256 + lane*4: v_perm takes byte
+    // 0 from lane4 (selector 0) and byte 1 from code byte mm of cw (selector
+    // 4+mm); bytes 2,3 are zero (selector 0x0C).  With a compact alphabet the
+    // byte is the residue's slot.
+    const uint32_t addr =
+        __builtin_amdgcn_perm(cw, lane4, 0x0C0C0000u | ((4u + (uint32_t)mm) << 8));
+    const uint32_t* pq = (const uint32_t*)((const char*)lds + addr);
+    #pragma unroll
+    for (int r = 0; r < W; r++) M[mm][r] = pq[r * kSlot];
+  }
+}
+
+// Four characters against the window: relative word j lives in register
+// (Q + j) mod W, carries run from relative word 0 upward.
+template <int W, int Q>
+__device__ __forceinline__ void win_compute_group(uint32_t* rows, const uint32_t (&M)[4][W]) {
+  #pragma unroll
+  for (int mm = 0; mm < 4; mm++) {
+    uint32_t cin = 0;
+    #pragma unroll
+    for (int j = 0; j < W; j++) {
+      constexpr int kW = W;
+      const int r = (Q + j) % kW;
+      rows[r] = lcs_step(rows[r], M[mm][r], cin, &cin);
+    }
+  }
+}
+
+// Window widths up to this one read the next group's masks ahead.
+constexpr int kWinPrefetchMax = 8;
+
 template <int W, int NC, int Q>
 __device__ __forceinline__ bool win_block(WinPair<W>& s, uint32_t cb0) {
   constexpr int kSlot = NC * 64;  // dwords per Peq slot
   const uint32_t cb = cb0 + Q;
   if (cb >= s.nb) return false;
   uint32_t* const lds = s.lds;
-  uint32_t* const cst = lds + W * kSlot;
-  const uint32_t lane = s.lane;
   // This block's codes arrived during the previous block; fetch the next
   // block's now (past the last block the read stays inside the buffer's
   // 64-byte tail padding), and the planes of the word entering at the end.
-  const uint4 c0 = s.nc0, c1 = s.nc1;
+  uint32_t q[8] = {s.nc0.x, s.nc0.y, s.nc0.z, s.nc0.w, s.nc1.x, s.nc1.y, s.nc1.z, s.nc1.w};
   s.nc0 = *(const uint4*)(s.tc + 8 * (cb + 1));
   s.nc1 = *(const uint4*)(s.tc + 8 * (cb + 1) + 4);
   const int32_t w = (int32_t)cb + s.c;
@@ -357,29 +412,33 @@ __device__ __forceinline__ bool win_block(WinPair<W>& s, uint32_t cb0) {
     const uint32_t* pl = s.rr + 1 + 4 * w;
     p0 = pl[0]; p1 = pl[1]; p2 = pl[2]; p3 = pl[3];
   }
-  // Stage this block's 32 target codes (4 per dword) in LDS.
-  cst[0 * 64 + lane] = c0.x; cst[1 * 64 + lane] = c0.y;
-  cst[2 * 64 + lane] = c0.z; cst[3 * 64 + lane] = c0.w;
-  cst[4 * 64 + lane] = c1.x; cst[5 * 64 + lane] = c1.y;
-  cst[6 * 64 + lane] = c1.z; cst[7 * 64 + lane] = c1.w;
-  for (int g = 0; g < 8; g++) {
-    const uint32_t cw = cst[g * 64 + lane];
-    #pragma unroll
-    for (int mm = 0; mm < 4; mm++) {
-      // byte address of Peq[code][lane] = code*256 + lane*4: v_perm takes
-      // byte 0 from lane4 (selector 0) and byte 1 from code byte mm of cw
-      // (selector 4+mm); bytes 2,3 are zero (selector 0x0C).  With a compact
-      // alphabet the byte is the residue's slot.
-      const uint32_t addr =
-          __builtin_amdgcn_perm(cw, s.lane4, 0x0C0C0000u | ((4u + (uint32_t)mm) << 8));
-      const uint32_t* pq = (const uint32_t*)((const char*)lds + addr);
-      uint32_t cin = 0;
+  if constexpr (W <= kWinPrefetchMax) {
+    // Eight groups of four characters.  The masks of the next group are read
+    // while the current group is computed (two buffers), so only the block's
+    // first group waits for LDS; the code words stay in registers, consumed
+    // two groups per step.
+    uint32_t MA[4][W], MB[4][W];
+    win_load_group<W, NC>(lds, q[0], s.lane4, MA);
+    #pragma unroll 1
+    for (int h = 0; h < 4; h++) {
+      win_load_group<W, NC>(lds, q[1], s.lane4, MB);
+      win_compute_group<W, Q>(s.rows, MA);
+      if (h < 3) win_load_group<W, NC>(lds, q[2], s.lane4, MA);
+      win_compute_group<W, Q>(s.rows, MB);
       #pragma unroll
-      for (int j = 0; j < W; j++) {
-        constexpr int kW = W;
-        const int r = (Q + j) % kW;  // register and Peq slot of relative word j
-        s.rows[r] = lcs_step(s.rows[r], pq[r * kSlot], cin, &cin);
-      }
+      for (int k = 0; k < 6; k++) q[k] = q[k + 2];
+    }
+  } else {
+    // Wide windows: one group per step, its code word staged in LDS, so that
+    // the round (unrolled W times) stays small enough for the instruction
+    // cache; two buffers of 4W masks would double it.
+    uint32_t* const cst = lds + W * kSlot;
+    #pragma unroll
+    for (int k = 0; k < 8; k++) cst[k * 64 + s.lane] = q[k];
+    for (int g = 0; g < 8; g++) {
+      uint32_t M[4][W];
+      win_load_group<W, NC>(lds, cst[g * 64 + s.lane], s.lane4, M);
+      win_compute_group<W, Q>(s.rows, M);
     }
   }
   // Slide: relative word 0 (register Q) leaves, word cb + c enters there.
@@ -387,7 +446,7 @@ __device__ __forceinline__ bool win_block(WinPair<W>& s, uint32_t cb0) {
   s.rows[Q] = 0xFFFFFFFFu;
   // Past the rep's last word the planes are 0, and so is every mask of a
   // nonzero code (each needs a set plane bit or a clear one with another set).
-  win_store_masks<NC>(lds + Q * kSlot, lane, p0, p1, p2, p3, s.slot_code);
+  win_store_masks<NC>(lds + Q * kSlot, s.lane, p0, p1, p2, p3, s.slot_code);
   return true;
 }
 
@@ -400,6 +459,10 @@ __device__ __forceinline__ void win_round(WinPair<W>& s, uint32_t cb0) {
 
 // NC = 16 indexes the Peq table by residue code; NC < 16 by the slot of a
 // compact alphabet (A.slots), which shrinks the table and raises occupancy.
+// Each lane holds one pair and takes the next from the work queue (one
+// atomic per wavefront for the lanes that need one) when its pair ends --
+// at the end of a round of W blocks, where the registers are back in their
+// starting order -- or when the pair can no longer pass (cannot_pass).
 template <int W, int NC>
 __global__ __launch_bounds__(64) void k_align_win(AlignArgs A, const uint2* __restrict__ pairs,
                                                   const uint32_t* __restrict__ list,
@@ -408,9 +471,10 @@ __global__ __launch_bounds__(64) void k_align_win(AlignArgs A, const uint2* __re
                                                   uint32_t* __restrict__ work,
                                                   uint32_t* __restrict__ score) {
   constexpr int kSlot = NC * 64;
-  // Peq slots [W][NC codes][64 lanes] + staged target codes [8][64 lanes].
-  // Static so that its LDS address is a compile-time constant.
-  __shared__ uint32_t lds[W * kSlot + 8 * 64];
+  // Peq slots [W][NC codes][64 lanes] (then, for wide windows, the block's
+  // staged codes [8][64]).  Static so that its LDS address is a compile-time
+  // constant.
+  __shared__ uint32_t lds[W * kSlot + (W <= kWinPrefetchMax ? 0 : 8 * 64)];
   const uint32_t lane = threadIdx.x;
   const uint32_t lane4 = lane * 4;
   const uint8_t* const rcodes = NC == 16 ? A.codes : A.slots;
@@ -418,46 +482,80 @@ __global__ __launch_bounds__(64) void k_align_win(AlignArgs A, const uint2* __re
   for (int s = 0; s < W; s++) lds[s * kSlot + lane] = 0;  // code (slot) 0 never matches
   const uint32_t count = *count_ptr;
   const uint32_t* const lst = list + (off_ptr ? *off_ptr : 0u);
+  uint32_t rows[W];
+  #pragma unroll
+  for (int j = 0; j < W; j++) rows[j] = 0xFFFFFFFFu;
+  const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+  WinPair<W> st{rows, 0u, A.packed, (const uint32_t*)rcodes, 0, 0, 0u, lds, lane, lane4,
+                zero4, zero4, A.slot_code};
+  bool active = false, drained = false;
+  uint32_t p = 0, R = 0, T = 0, cb0 = 0;
+  uint64_t ran = 0;  // tally: reference tiles of the blocks run
   for (;;) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(work, 64u);
-    base = __shfl(base, 0);
-    if (base >= count) break;
-    const uint32_t idx = base + lane;
-    if (idx >= count) continue;
-    const uint32_t p = lst[idx];
-    const uint2 pr = pairs[p];
-    const uint32_t* rr = A.packed + A.word_off[pr.x];
-    const uint32_t R = rr[0];
-    const uint32_t T = A.packed[A.word_off[pr.y]];
-    const Band bd = band_edges(R, T, A.thr);
-    const int32_t a = (int32_t)(bd.lsft / 32);
-    const int32_t c = (int32_t)((bd.rsft + 31) / 32);
-    const int32_t nw = (int32_t)((R + 31) / 32);
-    const uint32_t nb = (T + 31) / 32;
-    const uint32_t* tc = (const uint32_t*)(rcodes + A.code_off[pr.y]);
-    // Window at block 0: relative word j <-> absolute j - a, register j, slot j.
-    // Words outside the record get the masks of all-zero planes: 0 for every
-    // nonzero code.
-    uint32_t rows[W];
-    #pragma unroll
-    for (int j = 0; j < W; j++) {
-      rows[j] = 0xFFFFFFFFu;
-      const int32_t w = j - a;
-      uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
-      if (w >= 0 && w < nw) {
-        const uint32_t* pl = rr + 1 + 4 * w;
-        p0 = pl[0]; p1 = pl[1]; p2 = pl[2]; p3 = pl[3];
+    const uint64_t want = __builtin_amdgcn_ballot_w64(!active && !drained);
+    if (want) {
+      const uint32_t first = (uint32_t)__builtin_ctzll(want);
+      uint32_t base = 0;
+      if (lane == first) base = atomicAdd(work, (uint32_t)__popcll(want));
+      base = __builtin_amdgcn_readlane(base, first);
+      if (!active && !drained) {
+        const uint32_t idx = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
+        if (idx >= count) {
+          drained = true;
+        } else {
+          p = lst[idx];
+          const uint2 pr = pairs[p];
+          const uint32_t* rr = A.packed + A.word_off[pr.x];
+          R = rr[0];
+          T = A.packed[A.word_off[pr.y]];
+          const Band bd = band_edges(R, T, A.thr);
+          const int32_t a = (int32_t)(bd.lsft / 32);
+          const int32_t nw = (int32_t)((R + 31) / 32);
+          const uint32_t* tc = (const uint32_t*)(rcodes + A.code_off[pr.y]);
+          // Window at block 0: relative word j <-> absolute j - a, register
+          // j, slot j.  Words outside the record get the masks of all-zero
+          // planes: 0 for every nonzero code.
+          #pragma unroll
+          for (int j = 0; j < W; j++) {
+            rows[j] = 0xFFFFFFFFu;
+            const int32_t w = j - a;
+            uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+            if (w >= 0 && w < nw) {
+              const uint32_t* pl = rr + 1 + 4 * w;
+              p0 = pl[0]; p1 = pl[1]; p2 = pl[2]; p3 = pl[3];
+            }
+            win_store_masks<NC>(lds + j * kSlot, lane, p0, p1, p2, p3, A.slot_code);
+          }
+          st.lcs = 0u;
+          st.rr = rr;
+          st.tc = tc;
+          st.c = (int32_t)((bd.rsft + 31) / 32);
+          st.nw = nw;
+          st.nb = (T + 31) / 32;
+          st.nc0 = *(const uint4*)tc;
+          st.nc1 = *(const uint4*)(tc + 4);
+          cb0 = 0;
+          active = true;
+        }
       }
-      win_store_masks<NC>(lds + j * kSlot, lane, p0, p1, p2, p3, A.slot_code);
     }
-    WinPair<W> st{rows, 0u, rr, tc, c, nw, nb, lds, lane, lane4,
-                  *(const uint4*)tc, *(const uint4*)(tc + 4), A.slot_code};
-    for (uint32_t cb0 = 0; cb0 < nb; cb0 += W) win_round<W, NC, 0>(st, cb0);
-    #pragma unroll
-    for (int j = 0; j < W; j++) st.lcs += 32u - __popc(rows[j]);
-    score[p] = pair_value(A, st.lcs, R, T);
+    if (__builtin_amdgcn_ballot_w64(active) == 0) break;
+    if (active) {
+      win_round<W, NC, 0>(st, cb0);
+      cb0 += W;
+      uint32_t lcs = st.lcs;
+      #pragma unroll
+      for (int j = 0; j < W; j++) lcs += 32u - __popc(rows[j]);
+      const bool end = cb0 >= st.nb;
+      const bool fail = !end && !A.raw_lcs && cannot_pass(lcs, 32u * cb0, R, T, A.thr);
+      if (end || fail) {
+        score[p] = fail ? 0u : pair_value(A, lcs, R, T);
+        if (A.tally) ran += ref_tiles(R, T, A.thr, end ? st.nb : cb0);
+        active = false;
+      }
+    }
   }
+  tally_add(A, ran);
 }
 
 // ------------------------------------------------ one wavefront per pair
@@ -525,6 +623,7 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
   // The wider the window, the longer the pair: such waves are the critical
   // path of the concurrent bins, so the SIMD issues them first.
   __builtin_amdgcn_s_setprio(K >= 4 ? 3 : K == 2 ? 2 : 1);
+  uint64_t ran = 0;  // tally (lane 0): reference tiles of the blocks run
   for (;;) {
     uint32_t it = 0;
     if (lane == 0) it = atomicAdd(work, 1u);
@@ -569,6 +668,8 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
       const uint32_t* pl = rr + 1 + 4 * c;
       q0 = pl[0]; q1 = pl[1]; q2 = pl[2]; q3 = pl[3];
     }
+    bool failed = false;
+    uint32_t ran_blocks = nb;
     for (uint32_t cb = 0; cb < nb; cb++) {
       const uint32_t nxt = (lane < 8 && cb + 1 < nb) ? tcw[8 * (cb + 1) + lane] : 0u;
       const uint32_t* pqb = peq + vs;
@@ -603,22 +704,32 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
           uint32_t s0[K];
           uint64_t cy;
           asm("v_add_co_u32 %0, %1, %2, %3" : "=v"(s0[0]), "=s"(cy) : "v"(rows[0]), "v"(rows[0] & M[0]));
-          #pragma unroll
-          for (int m = 1; m < K; m++)
-            asm("v_addc_co_u32 %0, %1, %2, %3, %4"
-                : "=v"(s0[m]), "=s"(cy) : "v"(rows[m]), "v"(rows[m] & M[m]), "s"(cy));
-          uint32_t all = s0[0];
-          #pragma unroll
-          for (int m = 1; m < K; m++) all &= s0[m];
-          const uint64_t G = cy;  // lanes whose chain carries out
-          const uint64_t P = __builtin_amdgcn_ballot_w64(all == 0xFFFFFFFFu);
-          uint64_t ci = ((G | P) + G) ^ P;  // carry into each lane
-          #pragma unroll
-          for (int m = 0; m < K; m++) {
+          if constexpr (K == 1) {
+            const uint64_t G = cy;  // lanes whose word carries out
+            const uint64_t P = __builtin_amdgcn_ballot_w64(s0[0] == 0xFFFFFFFFu);
+            uint64_t ci = ((G | P) + G) ^ P;  // carry into each lane
             uint32_t s;
-            asm("v_addc_co_u32 %0, %1, %2, 0, %3" : "=v"(s), "=s"(ci) : "v"(s0[m]), "s"(ci));
-            rows[m] = s | (rows[m] & ~M[m]);
-            asm("" : "+v"(rows[m]));  // as in lcs_step: one bitop3, not two
+            asm("v_addc_co_u32 %0, %1, %2, 0, %3" : "=v"(s), "=s"(ci) : "v"(s0[0]), "s"(ci));
+            rows[0] = s | (rows[0] & ~M[0]);
+            asm("" : "+v"(rows[0]));  // as in lcs_step: one bitop3, not two
+          } else {
+            #pragma unroll
+            for (int m = 1; m < K; m++)
+              asm("v_addc_co_u32 %0, %1, %2, %3, %4"
+                  : "=v"(s0[m]), "=s"(cy) : "v"(rows[m]), "v"(rows[m] & M[m]), "s"(cy));
+            uint32_t all = s0[0];
+            #pragma unroll
+            for (int m = 1; m < K; m++) all &= s0[m];
+            const uint64_t G = cy;
+            const uint64_t P = __builtin_amdgcn_ballot_w64(all == 0xFFFFFFFFu);
+            uint64_t ci = ((G | P) + G) ^ P;
+            #pragma unroll
+            for (int m = 0; m < K; m++) {
+              uint32_t s;
+              asm("v_addc_co_u32 %0, %1, %2, 0, %3" : "=v"(s), "=s"(ci) : "v"(s0[m]), "s"(ci));
+              rows[m] = s | (rows[m] & ~M[m]);
+              asm("" : "+v"(rows[m]));
+            }
           }
         }
       }
@@ -647,22 +758,218 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
       }
       cur = nxt;
       asm volatile("" : "+v"(cur));
+      // Every 8 blocks: a pair that cannot pass any more ends with score 0.
+      if (!A.raw_lcs && (cb & 7u) == 7u && cb + 1 < nb) {
+        uint32_t part = 0;
+        #pragma unroll
+        for (int m = 0; m < K; m++) part += 32u - __popc(rows[m]);
+        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+        if (cannot_pass(lcs + part, 32u * (cb + 1), R, T, A.thr)) {
+          failed = true;
+          ran_blocks = cb + 1;
+          break;
+        }
+      }
     }
     uint32_t part = 0;
     #pragma unroll
     for (int m = 0; m < K; m++) part += 32u - __popc(rows[m]);
     for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
     lcs += part;
-    if (lane == 0) score[p] = pair_value(A, lcs, R, T);
+    if (lane == 0) score[p] = failed ? 0u : pair_value(A, lcs, R, T);
+    if (A.tally && lane == 0) ran += ref_tiles(R, T, A.thr, ran_blocks);
     lds_order();
   }
+  tally_add(A, ran);
+}
+
+// ------------------------------------------------ several pairs per wavefront
+// Band windows of at most G - 1 words (G = 16 or 32): one pair per G-lane
+// segment (64 / G pairs per wavefront), one word per lane.  In the segment of
+// lanes [base, base + G), lanes base .. base + G - 2 hold the words
+// [B, B + G - 1) of its pair's rep, B = cb - a, the band bottom; the band
+// [cb - a, cb + c) moves up one word per block, so a segment's registers
+// shift down one lane after every block (K = 1 of the wavefront kernel).
+// Lane base + G - 1 is a sentinel whose row restarts at 0 each block: within
+// a block of 32 characters that row stays below 2^31 (a row at most doubles
+// plus one per character), so it neither generates nor propagates a carry.
+// One 64-lane carry lookahead per character thus serves every segment, and
+// no carry crosses from one segment into the next.  The sentinel's slot is
+// the one the word that left last held (cleared), so its mask is 0.
+// Each segment takes the next pair from the work queue when its pair ends;
+// the segments run their blocks in lockstep, one without work holding code 0
+// (no change), and the wave leaves when every segment found the queue empty.
+template <int G>
+__global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __restrict__ pairs,
+                                                  const uint32_t* __restrict__ list,
+                                                  const uint32_t* __restrict__ off_ptr,
+                                                  const uint32_t* __restrict__ count_ptr,
+                                                  uint32_t* __restrict__ work,
+                                                  uint32_t* __restrict__ score) {
+  static_assert(G == 16 || G == 32, "segments of 16 or 32 lanes");
+  // [16 codes][64 slots]: the word w of the segment at lanes [base, base + G)
+  // has slot base + (w mod G); code 0 stays 0
+  __shared__ __align__(16) uint32_t peq[16 * 64];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t l = lane & (uint32_t)(G - 1), base = lane - l;
+  const uint32_t count = *count_ptr;
+  const uint32_t* const lst = list + (off_ptr ? *off_ptr : 0u);
+  __builtin_amdgcn_s_setprio(1);
+  for (uint32_t x = lane; x < 16 * 64; x += 64) peq[x] = 0u;
+  lds_order();
+  // The state of this lane's segment (equal in its G lanes; lcs in lane base).
+  bool active = false, need = true;
+  uint32_t p = 0, R = 0, T = 0, nb = 0, cb = 0, lcs = 0;
+  int32_t a = 0, c = 0, nw = 0;
+  const uint32_t* rr = A.packed;
+  const uint4* tcq = (const uint4*)A.codes;  // the target's codes, 32 bytes per block
+  uint32_t row = 0, vs = base, q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+  uint4 c0 = make_uint4(0u, 0u, 0u, 0u), c1 = c0;  // this block's 32 codes, one byte each
+  uint64_t ran = 0;  // tally (lane base): reference tiles of the blocks run
+  for (;;) {
+    // A segment whose pair ended takes the next (a pair without characters
+    // scores at once); one that finds the queue empty stays idle.
+    while (need) {
+      uint32_t it = 0;
+      if (l == 0) it = atomicAdd(work, 1u);
+      it = __shfl(it, (int)base);
+      need = false;
+      if (it >= count) {
+        active = false;
+        break;
+      }
+      p = lst[it];
+      const uint2 pr = pairs[p];
+      rr = A.packed + A.word_off[pr.x];
+      R = rr[0];
+      T = A.packed[A.word_off[pr.y]];
+      const Band bd = band_edges(R, T, A.thr);
+      a = (int32_t)(bd.lsft / 32);
+      c = (int32_t)((bd.rsft + 31) / 32);
+      nw = (int32_t)((R + 31) / 32);
+      nb = (T + 31) / 32;
+      tcq = (const uint4*)(A.codes + A.code_off[pr.y]);
+      if (nb == 0) {
+        if (l == 0) score[p] = pair_value(A, 0u, R, T);
+        need = true;
+        continue;
+      }
+      // The segment's slots: cleared, then the words in the band at block 0,
+      // [0, min(c, nw)) with c <= G - 1, word w at slot base + w.
+      #pragma unroll
+      for (int k = 1; k < 16; k++) peq[k * 64 + lane] = 0u;
+      lds_order();
+      const int32_t top0 = c < nw ? c : nw;
+      if ((int32_t)l < top0) {
+        const uint32_t* pl = rr + 1 + 4 * l;
+        uint32_t m[16];
+        peq_masks(pl[0], pl[1], pl[2], pl[3], m);
+        #pragma unroll
+        for (int k = 1; k < 16; k++) peq[k * 64 + lane] = m[k];
+      }
+      lds_order();
+      row = l < (uint32_t)(G - 1) ? 0xFFFFFFFFu : 0u;
+      vs = base + ((uint32_t)((int32_t)l - a) & (uint32_t)(G - 1));  // slot of word B + l, B = -a
+      lcs = 0;
+      cb = 0;
+      c0 = tcq[0];
+      c1 = tcq[1];
+      q0 = q1 = q2 = q3 = 0;
+      if (c < nw) {
+        const uint32_t* pl = rr + 1 + 4 * c;
+        q0 = pl[0]; q1 = pl[1]; q2 = pl[2]; q3 = pl[3];
+      }
+      active = true;
+    }
+    if (__builtin_amdgcn_ballot_w64(active) == 0) break;
+    // This block's codes (held in registers by every lane of the segment);
+    // the next block's are loaded now, to arrive during this one.
+    const uint32_t cw[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+    if (active && cb + 1 < nb) {
+      c0 = tcq[2 * (cb + 1)];
+      c1 = tcq[2 * (cb + 1) + 1];
+    } else {
+      c0 = c1 = make_uint4(0u, 0u, 0u, 0u);
+    }
+    const uint32_t vs4 = vs * 4;
+    // The masks of group g's four characters; those of group g + 1 are
+    // loaded while group g is computed (two buffers, g mod 2).
+    uint32_t Mb[2][4];
+    auto load_group = [&](int g, uint32_t (&Mx)[4]) {
+      #pragma unroll
+      for (int mm = 0; mm < 4; mm++) {
+        // byte address code * 256 + slot * 4 (slot < 64)
+        const uint32_t addr =
+            __builtin_amdgcn_perm(cw[g], vs4, 0x0C0C0000u | ((4u + (uint32_t)mm) << 8));
+        Mx[mm] = *(const uint32_t*)((const char*)peq + addr);
+      }
+    };
+    load_group(0, Mb[0]);
+    #pragma unroll
+    for (int g = 0; g < 8; g++) {
+      uint32_t(&Mq)[4] = Mb[g & 1];
+      if (g < 7) load_group(g + 1, Mb[(g + 1) & 1]);
+      #pragma unroll
+      for (int mm = 0; mm < 4; mm++) {
+        const uint32_t M = Mq[mm];
+        // As in k_align_wave: the carry chain through a lane mask (the carry
+        // operand of v_add_co / v_addc_co) and the ballot lookahead.
+        uint32_t s0;
+        uint64_t cy;
+        asm("v_add_co_u32 %0, %1, %2, %3" : "=v"(s0), "=s"(cy) : "v"(row), "v"(row & M));
+        const uint64_t G1 = cy;
+        const uint64_t P = __builtin_amdgcn_ballot_w64(s0 == 0xFFFFFFFFu);
+        uint64_t ci = ((G1 | P) + G1) ^ P;
+        uint32_t sum;
+        asm("v_addc_co_u32 %0, %1, %2, 0, %3" : "=v"(sum), "=s"(ci) : "v"(s0), "s"(ci));
+        row = sum | (row & ~M);
+        asm("" : "+v"(row));
+      }
+    }
+    // Word cb - a leaves the band, word cb + c enters it for block cb + 1.
+    if (active) {
+      const int32_t wl = (int32_t)cb - a, we = (int32_t)cb + c;
+      if (l >= 1 && l < 16) {
+        if (wl >= 0 && wl < nw) peq[l * 64 + base + ((uint32_t)wl & (uint32_t)(G - 1))] = 0u;
+        if (we < nw)
+          peq[l * 64 + base + ((uint32_t)we & (uint32_t)(G - 1))] = code_mask(q0, q1, q2, q3, l);
+      }
+      lds_order();
+      if (we + 1 < nw) {
+        const uint32_t* pl = rr + 1 + 4 * (we + 1);
+        q0 = pl[0]; q1 = pl[1]; q2 = pl[2]; q3 = pl[3];
+      }
+    }
+    // B moves up one word: the word in lane base leaves (counted there), the
+    // rows shift down one lane, lane G - 2 takes the next (all-ones) word and
+    // the sentinel restarts at 0.
+    if (l == 0) lcs += 32u - __popc(row);
+    const uint32_t up = __shfl_down(row, 1);
+    row = l == (uint32_t)(G - 2) ? 0xFFFFFFFFu : (l == (uint32_t)(G - 1) ? 0u : up);
+    vs = base + ((vs + 1u) & (uint32_t)(G - 1));
+    cb++;
+    // At the pair's end: the LCS summed over the segment (the words that left
+    // are counted in lane base).  No early exit here: a test every 8 blocks
+    // cost more than the pairs it ended saved (these pairs are short).
+    if (active && cb == nb) {
+      uint32_t part = (l < (uint32_t)(G - 1) ? 32u - __popc(row) : 0u) + (l == 0 ? lcs : 0u);
+      for (int off = G / 2; off > 0; off >>= 1) part += __shfl_xor(part, off);
+      if (l == 0) score[p] = pair_value(A, part, R, T);
+      if (A.tally && l == 0) ran += ref_tiles(R, T, A.thr, nb);
+      active = false;
+      need = true;
+    }
+  }
+  lds_order();
+  tally_add(A, ran);
 }
 
 // ------------------------------------------------ launchers for the bins
 size_t win_lds_bytes(int W, bool compact) {
-  return (size_t)W * (compact ? kSlotCodes + 1 : 16) * 256 + 8 * 256;
+  return (size_t)W * (compact ? kSlotCodes + 1 : 16) * 256 + (W <= kWinPrefetchMax ? 0 : 8 * 256);
 }
 size_t wave_lds_bytes(int K) { return (size_t)16 * 64 * K * 4; }
+size_t seg_lds_bytes() { return (size_t)16 * 64 * 4; }
 
 #define NGIA_WIN_CASE(Wv)                                                                  \
   case Wv:                                                                                 \
@@ -711,6 +1018,20 @@ int align_kernels_setup(size_t lds_per_block) {
     return 8;
   }
   return kWaveKMax;
+}
+
+void launch_align_seg(int G, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
+                      const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work,
+                      uint32_t* score, uint32_t blocks, hipStream_t stream) {
+  if (G == 16)
+    hipLaunchKernelGGL(k_align_seg<16>, dim3(blocks), dim3(64), 0, stream, a, pairs, list, off_ptr,
+                       count_ptr, work, score);
+  else if (G == 32)
+    hipLaunchKernelGGL(k_align_seg<32>, dim3(blocks), dim3(64), 0, stream, a, pairs, list, off_ptr,
+                       count_ptr, work, score);
+  else
+    throw HipError{NGIA_ERR_ARG, "no segmented kernel for G=" + std::to_string(G)};
+  NGIA_HIP(hipGetLastError());
 }
 
 #define NGIA_WAVE_CASE(Kv)                                                                   \

@@ -50,6 +50,23 @@ __host__ __device__ inline Band band_edges(uint32_t R, uint32_t T, float thr) {
 __host__ __device__ inline bool length_rejects(uint32_t R, uint32_t T, float thr) {
   return (float)T < (float)R * thr;
 }
+// Reference tiles (one rep word by one 32-character block, 1,024 cells) of
+// the first `blocks` column blocks of a pair: the word range the reference
+// updates for block cb (buildGraph.hpp:213-216), [max(32cb - lsft, 0),
+// min(32cb + rsft, R)), in words.
+__host__ __device__ inline uint64_t ref_tiles(uint32_t R, uint32_t T, float thr,
+                                              uint32_t blocks) {
+  const Band bd = band_edges(R, T, thr);
+  uint64_t t = 0;
+  for (uint32_t cb = 0; cb < blocks; cb++) {
+    int32_t js = (int32_t)(32 * cb) - (int32_t)bd.lsft;
+    if (js < 0) js = 0;
+    int32_t je = (int32_t)(32 * cb) + (int32_t)bd.rsft;
+    if (je > (int32_t)R) je = (int32_t)R;
+    if (je > js) t += ((uint32_t)(je - js) + 31) / 32;
+  }
+  return t;
+}
 // Pass test and integer percent score (buildGraph.hpp:248-252).
 __host__ __device__ inline uint32_t lcs_score(uint32_t lcs, uint32_t R, uint32_t T, float thr) {
   const uint32_t d = R + T - lcs;
@@ -103,7 +120,15 @@ struct AlignArgs {
   // full 16-code alphabet.
   const uint8_t* slots = nullptr;
   uint32_t slot_code = 0;
+  // Non-null: the kernel adds the reference tiles of the column blocks it
+  // ran (an early exit skips the rest of a pair's) here, one atomic per lane
+  // at its end.  Measurement only; nullptr in the product's rounds.
+  unsigned long long* tally = nullptr;
 };
+// The tally of a lane's alignments, added once when the lane is done.
+__device__ inline void tally_add(const AlignArgs& A, uint64_t ran) {
+  if (A.tally && ran) atomicAdd(A.tally, (unsigned long long)ran);
+}
 constexpr int kSlotCodes = 5;
 // What an alignment kernel stores for a pair it aligned: the score, or the
 // LCS length itself when raw_lcs is set (a check of the recurrence that does
@@ -143,6 +168,12 @@ void launch_align_win(int W, const AlignArgs& a, const uint2* pairs, const uint3
 void launch_align_wave(int K, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
                        const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work, uint32_t* score,
                        uint32_t blocks, hipStream_t stream);
+// 64 / G pairs per wavefront, one per G-lane segment (G = 16, 32), for band
+// windows of at most G - 1 words; any alphabet (the 16-code table).
+size_t seg_lds_bytes();
+void launch_align_seg(int G, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
+                      const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work,
+                      uint32_t* score, uint32_t blocks, hipStream_t stream);
 // Raise the dynamic-LDS limit of the kernels that need more than 64 KiB;
 // returns the largest K of k_align_wave<K> the device accepts.
 int align_kernels_setup(size_t lds_per_block);

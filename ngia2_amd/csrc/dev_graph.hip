@@ -336,18 +336,9 @@ __device__ inline uint32_t pair_bin(uint32_t R, uint32_t T, float thr, uint32_t 
   }
   const Band bd = band_edges(R, T, thr);
   const uint32_t a = bd.lsft / 32, c = (bd.rsft + 31) / 32, W = a + c;
-  // reference tiles: words of [max(32cb - lsft, 0), min(32cb + rsft, R)) per block
-  uint64_t t = 0;
-  const uint32_t nb = (T + 31) / 32;
-  for (uint32_t cb = 0; cb < nb; cb++) {
-    int32_t js = (int32_t)(32 * cb) - (int32_t)bd.lsft;
-    if (js < 0) js = 0;
-    int32_t je = (int32_t)(32 * cb) + (int32_t)bd.rsft;
-    if (je > (int32_t)R) je = (int32_t)R;
-    if (je > js) t += ((uint32_t)(je - js) + 31) / 32;
-  }
-  *tiles = t;
+  *tiles = ref_tiles(R, T, thr, (T + 31) / 32);
   if (W <= (uint32_t)kWinMax) return W;  // W >= 2 whenever T > 0
+  if (W <= 31) return kBinSeg32;
   for (uint32_t k = 0, K = 1; K <= kmax; k++, K <<= 1)
     if (W + K - 1 <= 64u * (uint32_t)K) return kBinWave0 + k;
   return kBinPlain;
@@ -383,22 +374,23 @@ __global__ void k_bin_count(const uint2* __restrict__ pairs, const uint64_t* __r
   }
 }
 
-// Route small window bins to the wavefront kernel (remap[b] = the bin that
-// aligns bin b's pairs, counts and tiles moved along), then bin_off[b] =
-// exclusive prefix of bin_cnt; also resets the scatter cursors.
+// Route window bins by the tuned shapes (remap[b] = the bin that aligns bin
+// b's pairs, counts and tiles moved along), then bin_off[b] = exclusive prefix
+// of bin_cnt; also resets the scatter cursors.
 __global__ void k_bin_offsets(uint32_t* __restrict__ bin_cnt, uint32_t* __restrict__ bin_off,
                               uint32_t* __restrict__ cursor, uint32_t* __restrict__ remap,
                               unsigned long long* __restrict__ bin_tiles, RouteTable route) {
   if (threadIdx.x == 0) {
     for (int b = 0; b < kBins; b++) remap[b] = (uint32_t)b;
     for (int W = 2; W <= kWinMax; W++) {
-      if (bin_cnt[W] && bin_cnt[W] < route.route_below[W]) {
-        remap[W] = kBinWave0;  // k_align_wave<1>: W <= 64
-        bin_cnt[kBinWave0] += bin_cnt[W];
-        bin_tiles[kBinWave0] += bin_tiles[W];
-        bin_cnt[W] = 0;
-        bin_tiles[W] = 0;
-      }
+      if (!bin_cnt[W]) continue;
+      const uint32_t to = route_window_bin(route, (uint32_t)W, bin_cnt[W]);
+      if (to == (uint32_t)W) continue;
+      remap[W] = to;
+      bin_cnt[to] += bin_cnt[W];
+      bin_tiles[to] += bin_tiles[W];
+      bin_cnt[W] = 0;
+      bin_tiles[W] = 0;
     }
     uint32_t s = 0;
     for (int b = 0; b < kBins; b++) {

@@ -6,18 +6,44 @@
 namespace ngia {
 
 // Pair bins: 0 = scored 0 without alignment, 2..kWinMax = k_align_win<W>,
-// kBinWave0 + log2(K) = k_align_wave<K>, kBinPlain = the plain kernel.
+// kBinWave0 + log2(K) = k_align_wave<K>, kBinPlain = the plain kernel,
+// kBinSeg16 / kBinSeg32 = k_align_seg<16> / <32> (several pairs per
+// wavefront; band windows of kWinMax+1 .. 31 words go to kBinSeg32).
 constexpr int kBinZero = 0;
 constexpr int kBinWave0 = kWinMax + 1;  // kWinMax+1 .. +6 for K = 1..32
 constexpr int kBinPlain = kBinWave0 + 6;
-constexpr int kBins = kBinPlain + 1;
+constexpr int kBinSeg16 = kBinPlain + 1;
+constexpr int kBinSeg32 = kBinSeg16 + 1;
+constexpr int kBins = kBinSeg32 + 1;
 
-// Routing of small window bins, set by the autotuner: window bin W with
-// fewer than route_below[W] pairs is aligned by k_align_wave<1> instead (a
-// wavefront per pair finishes a few pairs sooner than one thread per pair).
+// Shapes a window bin (W = 2..kWinMax) can be aligned with, chosen per bin
+// size by the autotuner: one thread per pair (k_align_win<W>), 16 or 32 lanes
+// per pair (k_align_seg), or a wavefront per pair (k_align_wave<1>).
+constexpr uint32_t kShapeWindow = NGIA_SHAPE_WINDOW;
+constexpr uint32_t kShapeSeg16 = NGIA_SHAPE_SEG16;
+constexpr uint32_t kShapeSeg32 = NGIA_SHAPE_SEG32;
+constexpr uint32_t kShapeWave = NGIA_SHAPE_WAVE;
+constexpr int kTuneClasses = NGIA_TUNE_CLASSES;
+// Window bin W holding c pairs is aligned with shape[W][k], k the last size
+// class with class_size[k] <= c (class 0 below class_size[0]).  A zeroed table
+// keeps every window bin on k_align_win<W>.
 struct RouteTable {
-  uint32_t route_below[kWinMax + 1];
+  uint32_t class_size[kTuneClasses];
+  uint8_t shape[kWinMax + 1][kTuneClasses];
 };
+
+// The bin that aligns window bin W's c pairs under the routing table.
+__host__ __device__ inline uint32_t route_window_bin(const RouteTable& rt, uint32_t W, uint32_t c) {
+  uint32_t k = 0;
+  for (uint32_t j = 1; j < (uint32_t)kTuneClasses; j++)
+    if (rt.class_size[j] && rt.class_size[j] <= c) k = j;
+  switch (rt.shape[W][k]) {
+    case kShapeSeg16: return W <= 15 ? (uint32_t)kBinSeg16 : (uint32_t)kBinSeg32;
+    case kShapeSeg32: return (uint32_t)kBinSeg32;
+    case kShapeWave: return (uint32_t)kBinWave0;
+    default: return W;
+  }
+}
 
 void fill_u32(uint32_t* p, uint32_t v, uint64_t count, hipStream_t s);
 uint64_t scan_scratch_words(uint32_t n);

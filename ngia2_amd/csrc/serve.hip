@@ -461,7 +461,8 @@ void launch_batch(ngia_serve* sv, uint32_t k, const uint8_t* res, const uint64_t
                      sv->tables.as<unsigned long long>(), sv->cap_log2, sv->lengths.as<uint32_t>(),
                      sv->thr, ln.pairs.as<uint2>(), &dc->total_cand, ln.ncand.as<uint32_t>());
   NGIA_HIP(hipGetLastError());
-  // The length bins, one after another on the lane stream.
+  // The length bins, one after another on the lane stream (the snapshot's
+  // window kernels use the 16-code table, so its routing applies).
   const uint64_t* np = (const uint64_t*)&dc->total_cand;
   bin_pairs(ln.pairs.as<uint2>(), np, sv->lengths.as<uint32_t>(), sv->thr, ln.bin_of.as<uint32_t>(),
             dc->bin_cnt, dc->bin_off, dc->cursor, ln.list.as<uint32_t>(), dc->bin_tiles,
@@ -478,6 +479,12 @@ void launch_batch(ngia_serve* sv, uint32_t k, const uint8_t* res, const uint64_t
     launch_align_wave(1 << kk, aa, ln.pairs.as<uint2>(), ln.list.as<uint32_t>(), &dc->bin_off[bin],
                       &dc->bin_cnt[bin], &dc->work[bin], ln.score.as<uint32_t>(),
                       std::min<uint32_t>(Q * 64, persistent_blocks(sv->ctx, wave_lds_bytes(1 << kk))), s);
+  }
+  for (int G = 32; G >= 16; G -= 16) {
+    const int bin = G == 16 ? kBinSeg16 : kBinSeg32;
+    launch_align_seg(G, aa, ln.pairs.as<uint2>(), ln.list.as<uint32_t>(), &dc->bin_off[bin],
+                     &dc->bin_cnt[bin], &dc->work[bin], ln.score.as<uint32_t>(),
+                     std::min<uint32_t>(Q * 64, persistent_blocks(sv->ctx, seg_lds_bytes())), s);
   }
   for (int W = kWinMax; W >= 2; W--)
     launch_align_win(W, aa, ln.pairs.as<uint2>(), ln.list.as<uint32_t>(), &dc->bin_off[W],

@@ -1,13 +1,15 @@
-// tune.hip -- the autotuner of the batched form's small window bins.
+// tune.hip -- the autotuner of the batched form's window bins.
 //
-// A window bin (band window W = 2..16 words) is aligned by k_align_win<W>,
-// one thread per pair: the most work per instruction, but a lone pair takes
-// T x W serial word-characters on one lane.  k_align_wave<1> gives each pair
-// a wavefront, which finishes a handful of pairs much sooner.  For each W the
-// tuner times both kernels on synthetic pairs whose band window is exactly W
-// (equal-length random sequences) at several bin sizes and records the
-// smallest bin size at which the window kernel is no slower; smaller bins
-// are then routed to the wavefront kernel by k_bin_offsets.
+// A window bin (band window W = 2..16 words) can be aligned in four shapes:
+// k_align_win<W>, one thread per pair (the most work per instruction, but a
+// lone pair's T x W word-characters run serially on one lane and a bin needs
+// hundreds of thousands of pairs to fill the device); k_align_seg<16> and
+// <32>, four and two pairs per wavefront with one word per lane; and
+// k_align_wave<1>, a wavefront per pair.  For each W the tuner times every
+// shape on synthetic pairs whose band window is exactly W (equal-length
+// random sequences) at bin sizes of 16 to 65,536 pairs, and installs the
+// fastest shape per (W, size class); k_bin_offsets then moves each window
+// bin's pairs to the bin of its shape on the device.
 #include <random>
 
 #include "capi_internal.hpp"
@@ -16,9 +18,11 @@ using namespace ngia;
 
 namespace {
 
-constexpr uint32_t kTuneCounts[] = {16, 64, 256, 1024, 4096};
-constexpr int kNumCounts = 5;
-constexpr uint32_t kTunePairs = 4096;
+constexpr uint32_t kTuneCounts[] = {16, 64, 256, 1024, 4096, 16384, 65536};
+constexpr int kNumCounts = 7;
+static_assert(kNumCounts <= kTuneClasses, "size classes");
+constexpr uint32_t kTunePairs = 4096;  // distinct synthetic pairs; larger bins repeat them
+constexpr uint32_t kTuneList = 65536;
 
 // Band window of a pair of lengths R >= T.
 uint32_t window_of(uint32_t R, uint32_t T, float thr) {
@@ -40,17 +44,43 @@ float time_kernel(ngia_ctx* ctx, const std::function<void()>& launch, uint32_t* 
   return best;
 }
 
+// Whether shape k can align a window of W words.
+bool shape_fits(uint32_t k, uint32_t W) { return k != kShapeSeg16 || W <= 15; }
+
 }  // namespace
 
 extern "C" int ngia_get_tuning(ngia_ctx* ctx, ngia_tuning* out) {
   if (!ctx || !out) return fail(ctx, NGIA_ERR_ARG, "null argument");
-  for (int W = 0; W <= kWinMax; W++) out->route_below[W] = ctx->route.route_below[W];
+  std::memset(out, 0, sizeof(*out));
+  for (int k = 0; k < kTuneClasses; k++) out->class_size[k] = ctx->route.class_size[k];
+  for (int W = 0; W <= kWinMax; W++)
+    for (int k = 0; k < kTuneClasses; k++) {
+      out->shape[W][k] = ctx->route.shape[W][k];
+      out->shape_compact[W][k] = ctx->route_compact.shape[W][k];
+    }
   return NGIA_OK;
 }
 
 extern "C" int ngia_set_tuning(ngia_ctx* ctx, const ngia_tuning* t) {
   if (!ctx || !t) return fail(ctx, NGIA_ERR_ARG, "null argument");
-  for (int W = 0; W <= kWinMax; W++) ctx->route.route_below[W] = W >= 2 ? t->route_below[W] : 0u;
+  ngia::RouteTable rt{}, rc{};
+  for (int k = 0; k < kTuneClasses; k++) {
+    rt.class_size[k] = rc.class_size[k] = t->class_size[k];
+    if (k && t->class_size[k] && t->class_size[k] <= t->class_size[k - 1])
+      return fail(ctx, NGIA_ERR_ARG, "class sizes must ascend");
+  }
+  for (int W = 2; W <= kWinMax; W++)
+    for (int k = 0; k < kTuneClasses; k++) {
+      for (int tab = 0; tab < 2; tab++) {
+        const uint32_t sh = tab ? t->shape_compact[W][k] : t->shape[W][k];
+        if (sh > kShapeWave) return fail(ctx, NGIA_ERR_ARG, "unknown shape");
+        if (!shape_fits(sh, (uint32_t)W))
+          return fail(ctx, NGIA_ERR_ARG, "16-lane segments hold W <= 15");
+        (tab ? rc : rt).shape[W][k] = (uint8_t)sh;
+      }
+    }
+  ctx->route = rt;
+  ctx->route_compact = rc;
   return NGIA_OK;
 }
 
@@ -66,23 +96,28 @@ extern "C" int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* 
     static const char kAmino[] = "ACDEFGHIKLMNPQRSTVWY";
     DevBuf pairs, list, score, ctr;
     pairs.ensure(8ull * kTunePairs);
-    list.ensure(4ull * kTunePairs);
+    list.ensure(4ull * kTuneList);
     score.ensure(4ull * kTunePairs);
     ctr.ensure(64);  // [0] offset, [1] count, [2] work
     std::vector<uint2> hp(kTunePairs);
-    std::vector<uint32_t> hl(kTunePairs);
-    for (uint32_t i = 0; i < kTunePairs; i++) {
-      hp[i] = make_uint2(2 * i, 2 * i + 1);
-      hl[i] = i;
-    }
+    std::vector<uint32_t> hl(kTuneList);
+    for (uint32_t i = 0; i < kTunePairs; i++) hp[i] = make_uint2(2 * i, 2 * i + 1);
+    for (uint32_t i = 0; i < kTuneList; i++) hl[i] = i % kTunePairs;
     h2d(pairs.p, hp.data(), 8ull * kTunePairs, s);
-    h2d(list.p, hl.data(), 4ull * kTunePairs, s);
+    h2d(list.p, hl.data(), 4ull * kTuneList, s);
     uint32_t* off = ctr.as<uint32_t>();
     uint32_t* cnt = off + 1;
     uint32_t* work = off + 2;
     uint32_t np = 0;
-    ngia::RouteTable route{};
+    ngia::RouteTable route{}, route_compact{};
+    for (int k = 0; k < kNumCounts; k++) route.class_size[k] = route_compact.class_size[k] = kTuneCounts[k];
+    // Two passes: proteins (the 16-code table) and DNA, whose window kernels
+    // index a compact table (at most five codes) and fit more waves per CU.
+    for (int dna = 0; dna < 2; dna++)
     for (int W = 2; W <= kWinMax; W++) {
+      const char* alpha = dna ? "ACGT" : kAmino;
+      const uint32_t na = dna ? 4 : 20;
+      ngia::RouteTable& rt = dna ? route_compact : route;
       // lengths R >= T passing the length test with window exactly W: the
       // median R among T = R, R - 8, ..., R - 64
       std::vector<std::pair<uint32_t, uint32_t>> cand;
@@ -104,11 +139,11 @@ extern "C" int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* 
       std::vector<uint64_t> woff(nseq);
       std::vector<uint8_t> res(R);
       for (uint32_t i = 0; i < kTunePairs; i++) {
-        for (auto& ch : res) ch = (uint8_t)kAmino[rng() % 20];
+        for (auto& ch : res) ch = (uint8_t)alpha[rng() % na];
         woff[2 * i] = (rw + tw) * i;
         pack_record(res.data(), R, packed.data() + woff[2 * i]);
         for (uint32_t j = 0; j < T; j++)
-          if (rng() % 20 == 0) res[j] = (uint8_t)kAmino[rng() % 20];
+          if (rng() % 20 == 0) res[j] = (uint8_t)alpha[rng() % na];
         woff[2 * i + 1] = woff[2 * i] + rw;
         pack_record(res.data(), T, packed.data() + woff[2 * i + 1]);
       }
@@ -116,33 +151,45 @@ extern "C" int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* 
       db.ctx = ctx;
       db_upload(ctx, &db, packed.data(), packed.size(), woff.data(), nullptr, nseq);
       AlignArgs aa = align_args(&db, thr);
-      uint32_t route_below = 0;
-      bool found = false;
       for (int k = 0; k < kNumCounts; k++) {
         const uint32_t n = kTuneCounts[k];
         const uint32_t zero = 0;
         h2d(off, &zero, 4, s);
         h2d(cnt, &n, 4, s);
-        const float win_ms = time_kernel(ctx, [&] {
-          launch_align_win(W, aa, pairs.as<uint2>(), list.as<uint32_t>(), off, cnt, work,
-                           score.as<uint32_t>(), persistent_blocks(ctx, win_lds_bytes(W)), s);
-        }, work);
-        const float wave_ms = time_kernel(ctx, [&] {
-          launch_align_wave(1, aa, pairs.as<uint2>(), list.as<uint32_t>(), off, cnt, work,
-                            score.as<uint32_t>(), persistent_blocks(ctx, wave_lds_bytes(1)), s);
-        }, work);
-        if (!found && win_ms <= wave_ms) {
-          route_below = n;
-          found = true;
+        double ms[4] = {0, 0, 0, 0};
+        for (uint32_t sh = kShapeWindow; sh <= kShapeWave; sh++) {
+          if (!shape_fits(sh, (uint32_t)W)) continue;
+          ms[sh] = time_kernel(ctx, [&] {
+            if (sh == kShapeWindow)
+              launch_align_win(W, aa, pairs.as<uint2>(), list.as<uint32_t>(), off, cnt, work,
+                               score.as<uint32_t>(), persistent_blocks(ctx, win_lds_bytes(W, aa.slots != nullptr)), s);
+            else if (sh == kShapeWave)
+              launch_align_wave(1, aa, pairs.as<uint2>(), list.as<uint32_t>(), off, cnt, work,
+                                score.as<uint32_t>(), persistent_blocks(ctx, wave_lds_bytes(1)), s);
+            else
+              launch_align_seg(sh == kShapeSeg16 ? 16 : 32, aa, pairs.as<uint2>(),
+                               list.as<uint32_t>(), off, cnt, work, score.as<uint32_t>(),
+                               persistent_blocks(ctx, seg_lds_bytes()), s);
+          }, work);
         }
-        if (points && np < cap) points[np] = ngia_tune_point{(uint32_t)W, L, n, win_ms, wave_ms};
+        uint32_t bestk = kShapeWindow;
+        for (uint32_t sh = kShapeSeg16; sh <= kShapeWave; sh++)
+          if (ms[sh] > 0 && ms[sh] < ms[bestk]) bestk = sh;
+        rt.shape[W][k] = (uint8_t)bestk;
+        if (points && np < cap) {
+          ngia_tune_point& pt = points[np];
+          pt.width = (uint32_t)W;
+          pt.dna = (uint32_t)dna;
+          pt.length = L;
+          pt.pairs = n;
+          for (int q = 0; q < 4; q++) pt.ms[q] = ms[q];
+        }
         np++;
       }
-      // the window kernel lost at every timed size: route up to 4x the largest
-      route.route_below[W] = found ? route_below : 4 * kTuneCounts[kNumCounts - 1];
     }
     NGIA_HIP(hipStreamSynchronize(s));
     ctx->route = route;
+    ctx->route_compact = route_compact;
     if (n_points) *n_points = np;
     return NGIA_OK;
   });

@@ -166,11 +166,15 @@ static void band_edges(uint32_t R, uint32_t T, float thr, int fused,
   *rsft = f2u_sat(ceilf(r)) + 32u;
 }
 
-/* B6: kernel_dynamic, buildGraph.hpp:195-252, as the same 32x32 tile loop. */
-uint32_t orc_align_lcs(const uint32_t* represent, const uint32_t* target, float thr, int fused) {
+/* B6: kernel_dynamic, buildGraph.hpp:195-252, as the same 32x32 tile loop.
+ * With lcs_out set, also the LCS after each 32-character block (the zero bits
+ * of all rows), and with tiles_out the tiles visited through it. */
+static uint32_t align_loop(const uint32_t* represent, const uint32_t* target, float thr,
+                           int fused, uint32_t* lcs_out, uint64_t* tiles_out) {
   const uint32_t R = represent[0], T = target[0];
   if ((float)T < (float)R * thr) return 0;  /* job keeps its initial 0 */
   const uint32_t nw = (R + 31) / 32;
+  uint64_t tiles = 0;
   uint32_t lines[2048];
   for (uint32_t w = 0; w < nw; w++) lines[w] = 0xFFFFFFFFu;
   uint32_t lsft, rsft;
@@ -206,11 +210,36 @@ uint32_t orc_align_lcs(const uint32_t* represent, const uint32_t* target, float 
         carrys = (carrys >> 1) | (carry << 31);
       }
       lines[j / 32] = row;
+      tiles++;
     }
+    if (lcs_out) {
+      uint32_t z = 0;
+      for (uint32_t w = 0; w < nw; w++) z += 32u - (uint32_t)__builtin_popcount(lines[w]);
+      lcs_out[i / 32] = z;
+    }
+    if (tiles_out) tiles_out[i / 32] = tiles;
   }
   uint32_t lcs = 0;
   for (uint32_t w = 0; w < nw; w++) lcs += 32u - (uint32_t)__builtin_popcount(lines[w]);
   return lcs;
+}
+
+uint32_t orc_align_lcs(const uint32_t* represent, const uint32_t* target, float thr, int fused) {
+  return align_loop(represent, target, thr, fused, NULL, NULL);
+}
+
+uint32_t orc_align_blocks(const uint32_t* represent, const uint32_t* target, float thr, int fused,
+                          uint32_t* lcs_out, uint64_t* tiles_out) {
+  const uint32_t R = represent[0], T = target[0];
+  if ((float)T < (float)R * thr) return 0;
+  align_loop(represent, target, thr, fused, lcs_out, tiles_out);
+  return (T + 31) / 32;
+}
+
+uint32_t orc_band_window(uint32_t R, uint32_t T, float thr, int fused) {
+  uint32_t lsft, rsft;
+  band_edges(R, T, thr, fused, &lsft, &rsft);
+  return lsft / 32 + (rsft + 31) / 32;
 }
 
 uint32_t orc_align_record(const uint32_t* represent, const uint32_t* target, float thr,

@@ -73,6 +73,15 @@ uint64_t orc_candidates(const uint32_t* rep_row, const uint32_t* len, uint32_t n
 uint32_t orc_align_record(const uint32_t* rep, const uint32_t* tar, float thr, int fused);
 /* The LCS length of the banded recurrence (0 when the length filter rejects). */
 uint32_t orc_align_lcs(const uint32_t* rep, const uint32_t* tar, float thr, int fused);
+/* The same loop, recording after each 32-character block k (k < ceil(T/32))
+ * the LCS so far, lcs_out[k], and the tiles visited through block k,
+ * tiles_out[k]; returns the block count (0 when the length filter rejects the
+ * pair, which is not aligned).  The LCS so far is that of the band-masked
+ * match matrix over the target's first 32(k+1) characters. */
+uint32_t orc_align_blocks(const uint32_t* rep, const uint32_t* tar, float thr, int fused,
+                          uint32_t* lcs_out, uint64_t* tiles_out);
+/* band window in words, lsft/32 + ceil(rsft/32) (buildGraph.hpp:208-216) */
+uint32_t orc_band_window(uint32_t R, uint32_t T, float thr, int fused);
 /* number of 32x32 tiles the reference loop visits for (R,T,thr) */
 uint64_t orc_align_tiles(uint32_t R, uint32_t T, float thr, int fused);
 /* kernel_dynamic contract: jobs[3i+2] = score of (jobs[3i], jobs[3i+1]) */

@@ -46,6 +46,10 @@ def orc():
         L.orc_align_lcs.restype = U32
         L.orc_align_lcs.argtypes = [P, P, C.c_float, C.c_int]
         L.orc_align_jobs_lcs.argtypes = [P, P, P, U32, C.c_float, C.c_int, C.c_int]
+        L.orc_align_blocks.restype = U32
+        L.orc_align_blocks.argtypes = [P, P, C.c_float, C.c_int, P, P]
+        L.orc_band_window.restype = U32
+        L.orc_band_window.argtypes = [U32, U32, C.c_float, C.c_int]
         L.orc_lcs_full.restype = U32
         L.orc_lcs_full.argtypes = [P, U32, P, U32]
         L.orc_build_graph.restype = C.c_int
@@ -147,6 +151,23 @@ def align_lcs(rep_rec, tar_rec, thr, fused=1):
 
 def align_tiles(R, T, thr, fused=1):
     return orc().orc_align_tiles(R, T, C.c_float(thr), fused)
+
+
+def align_blocks(rep_rec, tar_rec, thr, fused=1):
+    """Per 32-character block of the target: the LCS so far and the tiles
+    visited so far (two arrays of ceil(T/32); empty when the length filter
+    rejects the pair)."""
+    a = np.ascontiguousarray(rep_rec, dtype=np.uint32)
+    b = np.ascontiguousarray(tar_rec, dtype=np.uint32)
+    nb = (int(b[0]) + 31) // 32
+    lcs = np.zeros(max(nb, 1), dtype=np.uint32)
+    tiles = np.zeros(max(nb, 1), dtype=np.uint64)
+    k = orc().orc_align_blocks(_p(a), _p(b), C.c_float(thr), fused, _p(lcs), _p(tiles))
+    return lcs[:k], tiles[:k]
+
+
+def band_window(R, T, thr, fused=1):
+    return orc().orc_band_window(R, T, C.c_float(thr), fused)
 
 
 def align_jobs(packed, word_off, jobs, thr, fused=1, threads=0, raw_lcs=False):

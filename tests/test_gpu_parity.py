@@ -379,6 +379,7 @@ def test_band_edge_pairs(ctx):
                 db.close()
     waves = {w for k, w in kinds if k == "wave"}
     assert {1, 2, 4, 8, 16, 32} <= waves, kinds
+    assert ("seg", 32) in kinds, kinds
 
 
 def test_makedb_bytes(ctx, c1db, tmp_path):
@@ -486,40 +487,181 @@ def test_eager_batched_and_bin_accounting(ctx, longdb, identity):
     used = [b for b in bins if b["pairs"]]
     assert any(b["kind"] == "wave" for b in used) and any(b["kind"] == "window" for b in used)
     for b in used:
-        if b["kind"] in ("window", "wave", "plain"):
+        if b["kind"] in ("window", "wave", "seg", "plain"):
             assert b["ms"] > 0, b
 
 
-def test_routed_window_bins_score_alike(ctx, c1db):
-    """Window bins routed to the wavefront kernel (any bin size) score every pair
-    as the oracle does, and the bin accounting follows the pairs."""
-    _, _, d = c1db
-    say("routed window bins")
-    db = ng.Database(ctx, d["packed"], d["word_off"], d["lengths"])
-    db.set_reps(d["rep"], 64)
+@pytest.mark.parametrize("shape", [ng.SHAPE_SEG16, ng.SHAPE_SEG32, ng.SHAPE_WAVE])
+def test_routed_window_bins_score_alike(ctx, c1db, longdb, shape):
+    """Window bins routed to another shape (16 or 32 lanes per pair, or a
+    wavefront per pair) score every pair as the oracle does and as one thread
+    per pair does, and the bin accounting follows the pairs: the shape's bins
+    hold them and their tiles."""
+    say("routed window bins, shape", ng.SHAPES[shape])
     saved = ctx.tuning()
     try:
+        for name, (_, _, d) in (("c1", c1db), ("long", longdb)):
+            db = ng.Database(ctx, d["packed"], d["word_off"], d["lengths"])
+            db.set_reps(d["rep"], 64)
+            for identity in (50, 90):
+                ctx.route_all(ng.SHAPE_WINDOW)
+                db.graph(identity, 64, ng.GRAPH_BATCHED_EAGER)
+                keys0, sc0 = db.pair_scores()
+                edges0 = db.edges()
+                win0 = sum(b["pairs"] for b in db.bin_stats() if b["kind"] == "window")
+                ctx.route_all(shape)
+                st = db.graph(identity, 64, ng.GRAPH_BATCHED_EAGER)
+                keys, sc = db.pair_scores()
+                np.testing.assert_array_equal(keys, keys0)
+                np.testing.assert_array_equal(sc, sc0)
+                np.testing.assert_array_equal(db.edges(), edges0)
+                bins = db.bin_stats()
+                assert all(b["pairs"] == 0 for b in bins if b["kind"] == "window")
+                assert sum(b["ref_tiles"] for b in bins) == st["ref_tiles"]
+                if shape != ng.SHAPE_WAVE and win0:
+                    assert sum(b["pairs"] for b in bins if b["kind"] == "seg") >= win0
+                jobs = np.zeros((len(keys), 3), dtype=np.uint32)
+                jobs[:, 0] = keys >> np.uint64(32)
+                jobs[:, 1] = keys & np.uint64(0xFFFFFFFF)
+                thr = float(np.float32(identity) / np.float32(100))
+                np.testing.assert_array_equal(sc, orc.align_jobs(d["packed"], d["word_off"], jobs, thr)[:, 2])
+            db.close()
+    finally:
+        ctx.set_tuning(saved)
+
+
+def expected_run_tiles(R, T, thr, lcs, tiles, period):
+    """Reference tiles of the blocks a kernel that tests every `period` blocks
+    runs: it stops after block k (k a multiple of the period, k < the block
+    count) when the LCS so far plus the characters left fails the pass test
+    (buildGraph.hpp:248), else runs every block.  period 0: no test."""
+    nb = len(lcs)
+    thr32 = np.float32(thr)
+    if period:
+        for k in range(period, nb, period):
+            best = int(lcs[k - 1]) + max(T - 32 * k, 0)
+            if not np.float32(best) > np.float32(R + T - best) * thr32:
+                return int(tiles[k - 1])
+    return int(tiles[-1])
+
+
+def test_early_exit_tally_matches_oracle(ctx):
+    """The early exit, block for block: a tally round
+    (GRAPH_BATCHED_TALLY) scores every pair as the oracle does, and each bin's
+    kernel ran exactly the reference tiles the oracle's per-block LCS predicts
+    -- one thread per pair tests after every W blocks, a wavefront per pair
+    after every 8, the 32-lane segments not at all -- on the band-edge groups
+    (shifted copies, unrelated and low-complexity sequences fail part way)."""
+    rng = np.random.default_rng(71)
+    groups = band_edge_groups(rng)[:5]  # base lengths 90 .. 5000
+    seqs, gid = [], []
+    for k, g in enumerate(groups):
+        g = sorted(g, key=lambda x: -len(x))
+        seqs += g
+        gid += [k] * len(g)
+    recs = [orc.pack(x) for x in seqs]
+    off = np.cumsum([0] + [len(r) for r in recs[:-1]]).astype(np.uint64)
+    packed = np.concatenate(recs).astype(np.uint32)
+    lengths = np.array([len(x) for x in seqs], dtype=np.uint32)
+    n = len(seqs)
+    gid = np.array(gid)
+    rows = []
+    for b in range(64):
+        r = np.arange(n, dtype=np.int64) - 1 - b
+        ok = (r >= 0) & (gid[np.maximum(r, 0)] == gid)
+        r[~ok] = np.arange(n)[~ok]
+        rows.append(r.astype(np.uint32))
+    rep = np.stack(rows)
+    saved = ctx.tuning()
+    kinds = set()
+    try:
+        ctx.route_all(ng.SHAPE_WINDOW)
+        db = ng.Database(ctx, packed, off, lengths)
+        db.set_reps(rep, 64)
         for identity in (50, 90):
-            ctx.set_tuning([0] * 17)
-            db.graph(identity, 64, ng.GRAPH_BATCHED_EAGER)
-            keys0, sc0 = db.pair_scores()
-            edges0 = db.edges()
-            ctx.set_tuning([0, 0] + [1 << 30] * 15)  # every window bin -> k_align_wave<1>
-            st = db.graph(identity, 64, ng.GRAPH_BATCHED_EAGER)
+            say("early-exit tally, identity", identity, "records", n)
+            thr = float(np.float32(identity) / np.float32(100))
+            st = db.graph(identity, 64, ng.GRAPH_BATCHED_TALLY)
             keys, sc = db.pair_scores()
-            np.testing.assert_array_equal(keys, keys0)
-            np.testing.assert_array_equal(sc, sc0)
-            np.testing.assert_array_equal(db.edges(), edges0)
-            bins = db.bin_stats()
-            assert all(b["pairs"] == 0 for b in bins if b["kind"] == "window")
-            assert sum(b["ref_tiles"] for b in bins) == st["ref_tiles"]
             jobs = np.zeros((len(keys), 3), dtype=np.uint32)
             jobs[:, 0] = keys >> np.uint64(32)
             jobs[:, 1] = keys & np.uint64(0xFFFFFFFF)
-            thr = float(np.float32(identity) / np.float32(100))
-            np.testing.assert_array_equal(sc, orc.align_jobs(d["packed"], d["word_off"], jobs, thr)[:, 2])
+            np.testing.assert_array_equal(sc, orc.align_jobs(packed, off, jobs, thr)[:, 2])
+            want = {}
+            for u, v in jobs[:, :2]:
+                R, T = int(lengths[u]), int(lengths[v])
+                lcs, tiles = orc.align_blocks(recs[u], recs[v], thr)
+                if len(lcs) == 0:
+                    continue  # length-rejected: no kernel
+                W = orc.band_window(R, T, thr)
+                if W <= 16:
+                    key, period = ("window", W), W
+                elif W <= 31:
+                    key, period = ("seg", 32), 0
+                else:
+                    K = next(K for K in (1, 2, 4, 8, 16, 32) if W + K - 1 <= 64 * K)
+                    key, period = ("wave", K), 8
+                want[key] = want.get(key, 0) + expected_run_tiles(R, T, thr, lcs, tiles, period)
+            got = {(b["kind"], b["width"]): b["run_tiles"] for b in db.bin_stats() if b["pairs"]}
+            assert got == want, (identity, got, want)
+            bins = db.bin_stats()
+            assert sum(b["ref_tiles"] for b in bins) == st["ref_tiles"]
+            assert sum(b["run_tiles"] for b in bins) < st["ref_tiles"]  # some pairs ended early
+            kinds |= set(k for k, _ in want)
+        db.close()
     finally:
         ctx.set_tuning(saved)
+    assert kinds == {"window", "seg", "wave"}, kinds
+
+
+def test_segmented_kernels_raw_lcs(ctx):
+    """The segmented kernels at the LCS level: every window bin and the 32-lane
+    bin's own windows (17..31 words) of the band-edge groups, with raw LCS
+    lengths compared to the oracle's banded LCS (a band edge off by one word
+    changes only the LCS of pairs that fail)."""
+    rng = np.random.default_rng(57)
+    groups = band_edge_groups(rng)[:4]  # base lengths 90 .. 2300
+    seqs, gid = [], []
+    for k, g in enumerate(groups):
+        g = sorted(g, key=lambda x: -len(x))
+        seqs += g
+        gid += [k] * len(g)
+    recs = [orc.pack(x) for x in seqs]
+    off = np.cumsum([0] + [len(r) for r in recs[:-1]]).astype(np.uint64)
+    packed = np.concatenate(recs).astype(np.uint32)
+    lengths = np.array([len(x) for x in seqs], dtype=np.uint32)
+    n = len(seqs)
+    gid = np.array(gid)
+    rows = []
+    for b in range(64):
+        r = np.arange(n, dtype=np.int64) - 1 - b
+        ok = (r >= 0) & (gid[np.maximum(r, 0)] == gid)
+        r[~ok] = np.arange(n)[~ok]
+        rows.append(r.astype(np.uint32))
+    rep = np.stack(rows)
+    os.environ["NGIA_RAW_LCS"] = "1"
+    try:
+        raw_ctx = ng.Context(0)
+    finally:
+        del os.environ["NGIA_RAW_LCS"]
+    seg_pairs = 0
+    with raw_ctx:
+        for shape in (ng.SHAPE_SEG16, ng.SHAPE_SEG32):
+            raw_ctx.route_all(shape)
+            for identity in (30, 60, 90, 99):
+                say("segmented raw lcs, shape", ng.SHAPES[shape], "identity", identity)
+                thr = float(np.float32(identity) / np.float32(100))
+                db = ng.Database(raw_ctx, packed, off, lengths)
+                db.set_reps(rep, 64)
+                db.graph(identity, 64, ng.GRAPH_BATCHED)
+                keys, sc = db.pair_scores()
+                seg_pairs += sum(b["pairs"] for b in db.bin_stats() if b["kind"] == "seg")
+                jobs = np.zeros((len(keys), 3), dtype=np.uint32)
+                jobs[:, 0] = keys >> np.uint64(32)
+                jobs[:, 1] = keys & np.uint64(0xFFFFFFFF)
+                np.testing.assert_array_equal(sc, orc.align_jobs(packed, off, jobs, thr, raw_lcs=True)[:, 2])
+                db.close()
+    assert seg_pairs > 1000
 
 
 def test_captured_round_follows_tuning_and_rebucket(ctx, c1db):
@@ -536,20 +678,20 @@ def test_captured_round_follows_tuning_and_rebucket(ctx, c1db):
     other.set_reps(d["rep"], 64)
     saved = ctx.tuning()
     try:
-        ctx.set_tuning([0] * 17)
+        ctx.route_all(ng.SHAPE_WINDOW)
         db.sketch()
         db.bucket()
         db.graph(90, 64, ng.GRAPH_BATCHED)
         np.testing.assert_array_equal(db.edges(), want)
         win = sum(b["pairs"] for b in db.bin_stats() if b["kind"] == "window")
         assert win > 0
-        ctx.set_tuning([0, 0] + [1 << 30] * 15)  # every window bin -> k_align_wave<1>
+        ctx.route_all(ng.SHAPE_SEG16)  # every window bin -> four pairs per wavefront
         db.graph(90, 64, ng.GRAPH_BATCHED)
         np.testing.assert_array_equal(db.edges(), want)
         assert all(b["pairs"] == 0 for b in db.bin_stats() if b["kind"] == "window")
         other.graph(90, 64, ng.GRAPH_BATCHED)
         np.testing.assert_array_equal(other.edges(), want)
-        ctx.set_tuning([0] * 17)
+        ctx.route_all(ng.SHAPE_WINDOW)
         for _ in range(3):
             db.sketch()
             db.bucket()
@@ -564,18 +706,23 @@ def test_captured_round_follows_tuning_and_rebucket(ctx, c1db):
         db.close()
 
 
-def test_autotune_installs_crossovers(ctx):
+def test_autotune_installs_fastest_shapes(ctx):
+    """The tuner times every applicable shape per window width and bin size
+    (proteins, then DNA) and installs the fastest one per size class."""
     say("autotune")
     pts = ctx.autotune(90)
-    widths = sorted({p["width"] for p in pts})
-    assert widths == list(range(3, 17))  # W = 2 cannot pass the length test
-    assert all(p["window_ms"] > 0 and p["wave_ms"] > 0 for p in pts)
     t = ctx.tuning()
-    for W in widths:
-        mine = [p for p in pts if p["width"] == W]
-        won = [p["pairs"] for p in mine if p["window_ms"] <= p["wave_ms"]]
-        assert t[W] == (min(won) if won else 4 * 4096)
-    ctx.set_tuning([0] * 17)
+    assert t["class_size"][:7] == [16, 64, 256, 1024, 4096, 16384, 65536]
+    widths = sorted({p["width"] for p in pts if not p["dna"]})
+    assert widths == list(range(3, 17))  # W = 2 cannot pass the length test
+    for p in pts:
+        assert set(p["ms"]) == ({"window", "seg32", "wave"} | ({"seg16"} if p["width"] <= 15 else set()))
+        assert all(v > 0 for v in p["ms"].values())
+        k = [16, 64, 256, 1024, 4096, 16384, 65536].index(p["pairs"])
+        table = t["shape_compact"] if p["dna"] else t["shape"]
+        best = min(p["ms"], key=lambda s_: (p["ms"][s_], ng.SHAPES.index(s_)))
+        assert table[p["width"]][k] == ng.SHAPES.index(best), p
+    ctx.route_all(ng.SHAPE_WINDOW)
 
 
 @pytest.fixture(scope="module")

@@ -241,6 +241,79 @@ def test_align_lcs_whole_band_restatement():
     assert differs > 10  # the band bound the recurrence on many pairs
 
 
+def masked_chain_by_block(a, b, thr32):
+    """Longest chain of matches (strictly increasing in both sequences) in the
+    band-masked match matrix, over the target's first 32(k+1) characters for
+    each block k: the textbook LCS recurrence with match(j, t) true only where
+    the residues' codes agree (code 0 never matches) and rep residue j lies in
+    a word the reference updates for t's block (buildGraph.hpp:208-216)."""
+    R, T = len(a), len(b)
+    l32 = np.float32(T - T * np.float64(thr32))
+    r32 = np.float32(R - T * np.float64(thr32))
+    lsft = (max(int(np.ceil(l32)), 0) + 31) & ~31
+    rsft = max(int(np.ceil(r32)), 0) + 32
+    ca, cb_ = codes_of(a).astype(np.int64), codes_of(b).astype(np.int64)
+    row = np.zeros(R + 1, dtype=np.int64)  # row[j]: chain length over rep[:j]
+    out = []
+    pos = np.arange(R)
+    for t in range(T):
+        blk = t // 32
+        js, je = max(32 * blk - lsft, 0), min(32 * blk + rsft, R)
+        inband = (pos >= js) & (pos < 32 * ((je + 31) // 32)) if je > js else np.zeros(R, bool)
+        m = (ca == cb_[t]) & (ca != 0) & inband
+        cand = np.maximum(row[1:], row[:-1] + m)
+        row = np.concatenate([[0], np.maximum.accumulate(cand)])
+        if t % 32 == 31 or t == T - 1:
+            out.append(int(row[-1]))
+    return out, lsft // 32 + (rsft + 31) // 32
+
+
+def test_align_blocks_prefix_lcs_is_masked_chain():
+    """orc_align_blocks: after each block, the LCS so far equals the longest
+    chain of band-masked matches over the target read so far (a DP that shares
+    nothing with the bit-parallel loop), so it rises by at most one per target
+    character and the final LCS is at most the LCS so far plus the characters
+    left, the bound an early exit rests on; the last entry is orc_align_lcs,
+    the tiles' last entry orc_align_tiles, and orc_band_window the window
+    width of the same band edges."""
+    rng = np.random.default_rng(41)
+    exits = 0
+    for trial in range(160):
+        L = int(rng.integers(1, 200))
+        anc = bytes(AMINO[i] for i in rng.integers(20, size=L))
+        kind = trial % 4
+        if kind == 0:
+            b = bytes(AMINO[i] for i in rng.integers(20, size=int(rng.integers(1, L + 1))))
+        elif kind == 1:
+            sh = int(rng.integers(0, L // 2 + 1))
+            b = anc[sh:] + bytes(AMINO[i] for i in rng.integers(20, size=sh))
+        elif kind == 2:
+            b = b"".join(b"X" if rng.random() < 0.05 else anc[i:i + 1] for i in range(L))
+        else:
+            b = mutate(rng, anc, float(rng.uniform(0, 0.5)), float(rng.uniform(0, 0.1)))
+        a = anc
+        if len(b) > len(a):
+            a, b = b, a
+        thr = np.float32(int(rng.integers(1, 100))) / np.float32(100)
+        lcs, tiles = orc.align_blocks(orc.pack(a), orc.pack(b), float(thr))
+        R, T = len(a), len(b)
+        if np.float32(T) < np.float32(R) * thr:
+            assert len(lcs) == 0
+            continue
+        want, W = masked_chain_by_block(a, b, thr)
+        assert lcs.tolist() == want, (trial, R, T, float(thr))
+        assert orc.band_window(R, T, float(thr)) == W
+        assert lcs[-1] == orc.align_lcs(orc.pack(a), orc.pack(b), float(thr))
+        assert tiles[-1] == orc.align_tiles(R, T, float(thr))
+        done = np.minimum(32 * np.arange(1, len(lcs) + 1), T)
+        assert (np.diff(np.concatenate([[0], lcs])) <= np.diff(np.concatenate([[0], done]))).all()
+        assert (int(lcs[-1]) <= lcs.astype(np.int64) + (T - done)).all()
+        d = R + T - int(lcs[-1])
+        if len(lcs) > 1 and not np.float32(lcs[-1]) > np.float32(d) * thr:
+            exits += 1
+    assert exits > 20  # failing pairs (where an early exit can apply) were covered
+
+
 def test_fused_and_unfused_band_edges_score_alike():
     rng = np.random.default_rng(13)
     for trial in range(800):
