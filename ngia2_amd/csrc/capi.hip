@@ -222,7 +222,7 @@ void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
     mark(bin, 0, q);
     launch_align_seg(bin == kBinSeg16 ? 16 : 32, args(bin), db->pairs.as<uint2>(), db->list.as<uint32_t>(),
                      &dc->bin_off[bin], &dc->bin_cnt[bin], &dc->work[bin], db->score.as<uint32_t>(),
-                     persistent_blocks(ctx, seg_lds_bytes()), q);
+                     ctx->prop.multiProcessorCount * seg_blocks_per_cu(bin == kBinSeg16 ? 16 : 32), q);
     mark(bin, 1, q);
   };
   seg(kBinSeg32, fork ? ctx->side[0] : s);

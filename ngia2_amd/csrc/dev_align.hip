@@ -312,6 +312,41 @@ __device__ inline bool cannot_pass(uint32_t lcs_now, uint32_t done, uint32_t R, 
   return lcs_score(lcs_now + left, R, T, thr) == 0u;
 }
 
+// A wavefront's claim on its bin's work queue.  Every pair taken through a
+// per-pair atomic on the bin's one counter serialises a bin of many short
+// pairs on that address, so one atomic claims `chunk` consecutive items and
+// the wave hands them out in order.  The chunk follows the bin's size, so
+// each wave still gets many claims and the queue drains evenly:
+// count / (16 waves' worth), between `lo` and `hi` (lo >= the most items
+// handed out at once).  next/end are wave-uniform; take() is called by the
+// whole wave at a convergent point.
+struct WaveClaim {
+  uint32_t next = 0, end = 0, chunk = 1;
+  __device__ WaveClaim(uint32_t count, uint32_t lo, uint32_t hi) {
+    const uint32_t per = count / (gridDim.x * (blockDim.x / 64u) * 16u);
+    chunk = per < lo ? lo : per > hi ? hi : per;
+  }
+  // n (wave-uniform, n <= lo) lanes take an item each; `rank` (< n) is this
+  // lane's place among them.  Items from the counter's `count` on mean the
+  // queue is drained (every item below it is handed out exactly once: a wave
+  // hands out its claims in order and leaves only past `count`).
+  __device__ uint32_t take(uint32_t n, uint32_t rank, uint32_t* work) {
+    const uint32_t avail = end - next;
+    uint32_t item = next + rank;
+    if (n > avail) {
+      uint32_t b = 0;
+      if ((threadIdx.x & 63u) == 0) b = atomicAdd(work, chunk);
+      b = __builtin_amdgcn_readfirstlane(b);
+      if (rank >= avail) item = b + (rank - avail);
+      next = b + (n - avail);
+      end = b + chunk;
+    } else {
+      next += n;
+    }
+    return item;
+  }
+};
+
 // ------------------------------------------------ short window, thread/pair
 // Pair list entry: pairs[p] = (rep, tar); geometry computed by the binning
 // pass so that a + c == W for every pair of the list.
@@ -459,8 +494,8 @@ __device__ __forceinline__ void win_round(WinPair<W>& s, uint32_t cb0) {
 
 // NC = 16 indexes the Peq table by residue code; NC < 16 by the slot of a
 // compact alphabet (A.slots), which shrinks the table and raises occupancy.
-// Each lane holds one pair and takes the next from the work queue (one
-// atomic per wavefront for the lanes that need one) when its pair ends --
+// Each lane holds one pair and takes the next from the wave's claim on the
+// work queue (WaveClaim: chunks of 64 to 256 items) when its pair ends --
 // at the end of a round of W blocks, where the registers are back in their
 // starting order -- or when the pair can no longer pass (cannot_pass).
 template <int W, int NC>
@@ -491,15 +526,13 @@ __global__ __launch_bounds__(64) void k_align_win(AlignArgs A, const uint2* __re
   bool active = false, drained = false;
   uint32_t p = 0, R = 0, T = 0, cb0 = 0;
   uint64_t ran = 0;  // tally: reference tiles of the blocks run
+  WaveClaim q(count, 64u, 256u);
   for (;;) {
     const uint64_t want = __builtin_amdgcn_ballot_w64(!active && !drained);
     if (want) {
-      const uint32_t first = (uint32_t)__builtin_ctzll(want);
-      uint32_t base = 0;
-      if (lane == first) base = atomicAdd(work, (uint32_t)__popcll(want));
-      base = __builtin_amdgcn_readlane(base, first);
+      const uint32_t idx = q.take((uint32_t)__popcll(want),
+                                  (uint32_t)__popcll(want & ((1ull << lane) - 1ull)), work);
       if (!active && !drained) {
-        const uint32_t idx = base + (uint32_t)__popcll(want & ((1ull << lane) - 1ull));
         if (idx >= count) {
           drained = true;
         } else {
@@ -624,10 +657,9 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
   // path of the concurrent bins, so the SIMD issues them first.
   __builtin_amdgcn_s_setprio(K >= 4 ? 3 : K == 2 ? 2 : 1);
   uint64_t ran = 0;  // tally (lane 0): reference tiles of the blocks run
+  WaveClaim q(count, 1u, K == 1 ? 4u : 1u);
   for (;;) {
-    uint32_t it = 0;
-    if (lane == 0) it = atomicAdd(work, 1u);
-    it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+    const uint32_t it = q.take(1u, 0u, work);
     if (it >= count) break;
     const uint32_t p = lst[it];
     const uint2 pr = pairs[p];
@@ -796,7 +828,8 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
 // One 64-lane carry lookahead per character thus serves every segment, and
 // no carry crosses from one segment into the next.  The sentinel's slot is
 // the one the word that left last held (cleared), so its mask is 0.
-// Each segment takes the next pair from the work queue when its pair ends;
+// Each segment takes the next pair from the wave's claim on the work queue
+// (WaveClaim) when its pair ends;
 // the segments run their blocks in lockstep, one without work holding code 0
 // (no change), and the wave leaves when every segment found the queue empty.
 template <int G>
@@ -826,60 +859,65 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
   uint32_t row = 0, vs = base, q0 = 0, q1 = 0, q2 = 0, q3 = 0;
   uint4 c0 = make_uint4(0u, 0u, 0u, 0u), c1 = c0;  // this block's 32 codes, one byte each
   uint64_t ran = 0;  // tally (lane base): reference tiles of the blocks run
+  WaveClaim q(count, 64u / G, 4u * (64u / G));
   for (;;) {
-    // A segment whose pair ended takes the next (a pair without characters
-    // scores at once); one that finds the queue empty stays idle.
-    while (need) {
-      uint32_t it = 0;
-      if (l == 0) it = atomicAdd(work, 1u);
-      it = __shfl(it, (int)base);
-      need = false;
-      if (it >= count) {
+    // Segments whose pair ended take the next ones; one that finds the queue
+    // empty stays idle, and one whose pair has no characters scores it and
+    // takes another in the next pass.
+    const uint64_t want = __builtin_amdgcn_ballot_w64(need && l == 0);
+    if (want) {
+      const uint32_t it = __shfl(q.take((uint32_t)__popcll(want),
+                                        (uint32_t)__popcll(want & ((1ull << lane) - 1ull)), work),
+                                 (int)base);
+      if (need) {
+        need = false;
         active = false;
-        break;
+        if (it < count) {
+          p = lst[it];
+          const uint2 pr = pairs[p];
+          rr = A.packed + A.word_off[pr.x];
+          R = rr[0];
+          T = A.packed[A.word_off[pr.y]];
+          const Band bd = band_edges(R, T, A.thr);
+          a = (int32_t)(bd.lsft / 32);
+          c = (int32_t)((bd.rsft + 31) / 32);
+          nw = (int32_t)((R + 31) / 32);
+          nb = (T + 31) / 32;
+          tcq = (const uint4*)(A.codes + A.code_off[pr.y]);
+          if (nb == 0) {
+            if (l == 0) score[p] = pair_value(A, 0u, R, T);
+            need = true;
+          } else {
+            // The segment's slots: cleared, then the words in the band at
+            // block 0, [0, min(c, nw)) with c <= G - 1, word w at slot base + w.
+            #pragma unroll
+            for (int k = 1; k < 16; k++) peq[k * 64 + lane] = 0u;
+            lds_order();
+            const int32_t top0 = c < nw ? c : nw;
+            if ((int32_t)l < top0) {
+              const uint32_t* pl = rr + 1 + 4 * l;
+              uint32_t m[16];
+              peq_masks(pl[0], pl[1], pl[2], pl[3], m);
+              #pragma unroll
+              for (int k = 1; k < 16; k++) peq[k * 64 + lane] = m[k];
+            }
+            lds_order();
+            row = l < (uint32_t)(G - 1) ? 0xFFFFFFFFu : 0u;
+            vs = base + ((uint32_t)((int32_t)l - a) & (uint32_t)(G - 1));  // slot of word B + l, B = -a
+            lcs = 0;
+            cb = 0;
+            c0 = tcq[0];
+            c1 = tcq[1];
+            q0 = q1 = q2 = q3 = 0;
+            if (c < nw) {
+              const uint32_t* pl = rr + 1 + 4 * c;
+              q0 = pl[0]; q1 = pl[1]; q2 = pl[2]; q3 = pl[3];
+            }
+            active = true;
+          }
+        }
       }
-      p = lst[it];
-      const uint2 pr = pairs[p];
-      rr = A.packed + A.word_off[pr.x];
-      R = rr[0];
-      T = A.packed[A.word_off[pr.y]];
-      const Band bd = band_edges(R, T, A.thr);
-      a = (int32_t)(bd.lsft / 32);
-      c = (int32_t)((bd.rsft + 31) / 32);
-      nw = (int32_t)((R + 31) / 32);
-      nb = (T + 31) / 32;
-      tcq = (const uint4*)(A.codes + A.code_off[pr.y]);
-      if (nb == 0) {
-        if (l == 0) score[p] = pair_value(A, 0u, R, T);
-        need = true;
-        continue;
-      }
-      // The segment's slots: cleared, then the words in the band at block 0,
-      // [0, min(c, nw)) with c <= G - 1, word w at slot base + w.
-      #pragma unroll
-      for (int k = 1; k < 16; k++) peq[k * 64 + lane] = 0u;
-      lds_order();
-      const int32_t top0 = c < nw ? c : nw;
-      if ((int32_t)l < top0) {
-        const uint32_t* pl = rr + 1 + 4 * l;
-        uint32_t m[16];
-        peq_masks(pl[0], pl[1], pl[2], pl[3], m);
-        #pragma unroll
-        for (int k = 1; k < 16; k++) peq[k * 64 + lane] = m[k];
-      }
-      lds_order();
-      row = l < (uint32_t)(G - 1) ? 0xFFFFFFFFu : 0u;
-      vs = base + ((uint32_t)((int32_t)l - a) & (uint32_t)(G - 1));  // slot of word B + l, B = -a
-      lcs = 0;
-      cb = 0;
-      c0 = tcq[0];
-      c1 = tcq[1];
-      q0 = q1 = q2 = q3 = 0;
-      if (c < nw) {
-        const uint32_t* pl = rr + 1 + 4 * c;
-        q0 = pl[0]; q1 = pl[1]; q2 = pl[2]; q3 = pl[3];
-      }
-      active = true;
+      if (__builtin_amdgcn_ballot_w64(need) != 0) continue;
     }
     if (__builtin_amdgcn_ballot_w64(active) == 0) break;
     // This block's codes (held in registers by every lane of the segment);
@@ -1018,6 +1056,22 @@ int align_kernels_setup(size_t lds_per_block) {
     return 8;
   }
   return kWaveKMax;
+}
+
+uint32_t seg_blocks_per_cu(int G) {
+  // Resident single-wave blocks per CU as registers and LDS allow (8 per
+  // SIMD at 60 VGPRs and 4 KiB): twice the 16 that other bins get, which
+  // took 25% off a bin of 2.4M pairs at windows of 17-31 words.
+  static int per_cu[2] = {0, 0};
+  int& v = per_cu[G == 16 ? 0 : 1];
+  if (!v) {
+    int nb = 0;
+    const hipError_t e = G == 16
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_align_seg<16>, 64, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_align_seg<32>, 64, 0);
+    v = (e == hipSuccess && nb > 0) ? std::min(nb, 32) : 16;
+  }
+  return (uint32_t)v;
 }
 
 void launch_align_seg(int G, const AlignArgs& a, const uint2* pairs, const uint32_t* list,

@@ -484,7 +484,8 @@ void launch_batch(ngia_serve* sv, uint32_t k, const uint8_t* res, const uint64_t
     const int bin = G == 16 ? kBinSeg16 : kBinSeg32;
     launch_align_seg(G, aa, ln.pairs.as<uint2>(), ln.list.as<uint32_t>(), &dc->bin_off[bin],
                      &dc->bin_cnt[bin], &dc->work[bin], ln.score.as<uint32_t>(),
-                     std::min<uint32_t>(Q * 64, persistent_blocks(sv->ctx, seg_lds_bytes())), s);
+                     std::min<uint32_t>(Q * 64, sv->ctx->prop.multiProcessorCount * seg_blocks_per_cu(G)),
+                     s);
   }
   for (int W = kWinMax; W >= 2; W--)
     launch_align_win(W, aa, ln.pairs.as<uint2>(), ln.list.as<uint32_t>(), &dc->bin_off[W],

@@ -169,7 +169,8 @@ extern "C" int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* 
             else
               launch_align_seg(sh == kShapeSeg16 ? 16 : 32, aa, pairs.as<uint2>(),
                                list.as<uint32_t>(), off, cnt, work, score.as<uint32_t>(),
-                               persistent_blocks(ctx, seg_lds_bytes()), s);
+                               ctx->prop.multiProcessorCount *
+                                   seg_blocks_per_cu(sh == kShapeSeg16 ? 16 : 32), s);
           }, work);
         }
         uint32_t bestk = kShapeWindow;
