@@ -97,6 +97,9 @@ typedef struct {
   double graph_ms;    /* candidates .. raw edges of the last ngia_db_graph */
   double finalize_ms; /* sort + unique of the last ngia_db_graph */
   double align_ms;    /* alignment bins alone (NGIA_GRAPH_BATCHED_EAGER only, else 0) */
+  double step_ms;     /* the whole of the last ngia_db_step */
+  uint32_t step_head; /* records in the head round of the last ngia_db_step (0: none) */
+  uint32_t reserved;
 } ngia_timing;
 int ngia_get_timing(ngia_ctx* ctx, ngia_timing* out);
 void ngia_free(void* p);
@@ -191,6 +194,21 @@ int ngia_db_get_reps(ngia_ctx* ctx, ngia_db* db, uint32_t* rep);   /* 64 x n */
 /* Build the edge list on the device; it stays resident until the next call. */
 int ngia_db_graph(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
                   uint32_t flags, ngia_graph_stats* stats);
+
+/* Sketch, bucketing and the batched graph of the whole database in one call:
+ * the signatures, rows and sorted unique edges of ngia_db_sketch,
+ * ngia_db_bucket and ngia_db_graph(NGIA_GRAPH_BATCHED) in sequence.  No
+ * reference counterpart (the reference runs makeDB and clustering as two
+ * programs, main.cu:62-117).  The records whose own band window exceeds 31
+ * words at this identity (the leading run: makeDB orders records longest
+ * first) form a head; its graph, whose pairs depend on the head alone,
+ * runs beside the sketch of the rest, and the main round takes the other
+ * targets.  NGIA_STEP_NO_SPLIT runs the three stages one after another.
+ * Bin figures (ngia_db_bin_stats) and pair scores then cover the main
+ * round only. */
+#define NGIA_STEP_NO_SPLIT 1u
+int ngia_db_step(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands, uint32_t flags,
+                 ngia_graph_stats* st);
 int ngia_db_get_edges(ngia_ctx* ctx, ngia_db* db, ngia_edge* out, uint64_t cap,
                       uint64_t* n_edges);
 /* Scores of the distinct pairs of the last batched graph, sorted by (rep,tar):

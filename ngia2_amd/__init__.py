@@ -271,6 +271,17 @@ class Database:
                self.ctx.h)
         return st.as_dict()
 
+    def step(self, identity, bands=BANDS, split=True):
+        """Sketch, bucketing and the batched graph in one call (ngia_db_step):
+        the same signatures, rows and edges as sketch(), bucket() and
+        graph(identity, bands); with split, the head's graph (records whose
+        own band window exceeds 31 words) runs beside the rest's sketch."""
+        st = _lib.GraphStats()
+        _check(_lib.lib().ngia_db_step(self.ctx.h, self.h, identity, bands,
+                                       0 if split else _lib.STEP_NO_SPLIT, C.byref(st)),
+               self.ctx.h)
+        return st.as_dict()
+
     def edges(self):
         m = C.c_uint64()
         L = _lib.lib()

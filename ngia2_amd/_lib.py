@@ -21,6 +21,7 @@ GRAPH_PLAIN = 0
 GRAPH_BATCHED = 1
 GRAPH_BATCHED_EAGER = 2
 GRAPH_BATCHED_TALLY = 3
+STEP_NO_SPLIT = 1
 BANDS = 64
 
 
@@ -71,7 +72,8 @@ class TunePoint(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("sketch_ms", C.c_double), ("bucket_ms", C.c_double), ("graph_ms", C.c_double),
-                ("finalize_ms", C.c_double), ("align_ms", C.c_double)]
+                ("finalize_ms", C.c_double), ("align_ms", C.c_double), ("step_ms", C.c_double),
+                ("step_head", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class DeviceInfo(C.Structure):
@@ -190,6 +192,7 @@ PROTOTYPES = {
     "ngia_db_get_sigs": (C.c_int, [P, P, P]),
     "ngia_db_get_reps": (C.c_int, [P, P, P]),
     "ngia_db_graph": (C.c_int, [P, P, U32, U32, U32, C.POINTER(GraphStats)]),
+    "ngia_db_step": (C.c_int, [P, P, U32, U32, U32, C.POINTER(GraphStats)]),
     "ngia_db_get_edges": (C.c_int, [P, P, P, U64, C.POINTER(U64)]),
     "ngia_db_get_pair_scores": (C.c_int, [P, P, P, P, U64, C.POINTER(U64)]),
     "ngia_db_band_jobs": (C.c_int, [P, P, U32, U32, P, U64, C.POINTER(U64)]),

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -146,15 +147,15 @@ void graph_plain(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
 // wide window kernel (55-68 KiB of LDS per block) waits for CUs the busy bins
 // hold, stalling every kernel queued behind it on its stream.
 
-// Part 1: candidates and bins (capturable: fixed grids, device-side counts).
-void enqueue_bins(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands) {
-  hipStream_t s = ctx->stream;
+// Part 1: candidates and bins (capturable: fixed grids, device-side counts),
+// on stream s; targets from db->t0 on.
+void enqueue_bins(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands, hipStream_t s) {
   const uint32_t n = db->n;
   const float thr = (float)identity / 100.0f;
   Counters* dc = db->counters.as<Counters>();
   const uint32_t* len = db->lengths.as<uint32_t>();
   fill_u32(db->counters.as<uint32_t>(), 0, sizeof(Counters) / 4, s);
-  pair_count(db->rep.as<uint32_t>(), len, n, bands, thr, db->cnt.as<uint32_t>(),
+  pair_count(db->rep.as<uint32_t>(), len, n, db->t0, bands, thr, db->cnt.as<uint32_t>(),
              db->keep.as<uint64_t>(), &dc->total_cand, s);
   scan_u32(db->cnt.as<uint32_t>(), n, db->off.as<uint64_t>(), db->sums.as<uint64_t>(), s);
   pair_emit(db->rep.as<uint32_t>(), n, bands, db->keep.as<uint64_t>(), db->off.as<uint64_t>(),
@@ -167,10 +168,10 @@ void enqueue_bins(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands)
 }
 
 // Part 2: the alignment kernels of the bins in `mask` (bit b: bin b holds
-// pairs), then the passing pairs as raw edges (capturable).
+// pairs), then the passing pairs as raw edges (capturable), on stream s and,
+// unless `serial`, the side streams.
 void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
-                   bool time_align, bool tally = false) {
-  hipStream_t s = ctx->stream;
+                   bool time_align, bool tally, hipStream_t s, bool serial = false) {
   const uint32_t n = db->n;
   const float thr = (float)identity / 100.0f;
   Counters* dc = db->counters.as<Counters>();
@@ -196,7 +197,7 @@ void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
   // much free LDS, which the other bins' persistent blocks hold until they
   // finish.
   if (time_align) NGIA_HIP(hipEventRecord(ctx->ev[2], s));
-  const bool fork = !ctx->serial_bins;
+  const bool fork = !ctx->serial_bins && !serial;
   auto mark = [&](int bin, int end, hipStream_t q) {
     if (time_align) NGIA_HIP(hipEventRecord(ctx->bin_ev[2 * bin + end], q));
   };
@@ -310,11 +311,20 @@ bool ensure_plain_bin_space(ngia_db* db, uint32_t bands) {
   return db->plain_jobs.p != j0 || db->plain_scratch.p != s0;
 }
 
+// The head round of a step (ngia_db_step), whose raw edges join the main
+// round's before the sort: its database and the event recorded after them.
+struct HeadRound {
+  ngia_db* db = nullptr;
+  hipEvent_t done = nullptr;
+};
+
 // The batched round: its first part captured once per (identity, bands), its
 // second once per set of bins that hold pairs, both replayed (or launched
-// eagerly, with the alignment phase timed).
+// eagerly, with the alignment phase timed).  With a head round, the main
+// round's targets start at db->t0 and the head's raw edges are appended
+// before the sort.
 void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
-                   ngia_graph_stats* st, bool eager, bool tally) {
+                   ngia_graph_stats* st, bool eager, bool tally, const HeadRound* head = nullptr) {
   hipStream_t s = ctx->stream;
   const uint32_t n = db->n;
   ensure_batched_space(db, bands);
@@ -331,18 +341,19 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
     for (int W = 0; W <= kWinMax; W++)
       for (int k = 0; k < kTuneClasses; k++) key.push_back(rt.shape[W][k]);
   }
+  key.push_back(db->t0);
   const bool reuse = db->gexec && db->g_identity == identity && db->g_bands == bands &&
                      db->g_key == key;
   if (!eager && !reuse) {
     drop_graphs(db);
-    db->gexec = capture(s, [&] { enqueue_bins(ctx, db, identity, bands); });
+    db->gexec = capture(s, [&] { enqueue_bins(ctx, db, identity, bands, s); });
     db->g_identity = identity;
     db->g_bands = bands;
     db->g_key = key;
   }
   NGIA_HIP(hipEventRecord(ctx->ev[0], s));
   if (eager)
-    enqueue_bins(ctx, db, identity, bands);
+    enqueue_bins(ctx, db, identity, bands, s);
   else
     NGIA_HIP(hipGraphLaunch(db->gexec, s));
   // the bins that hold pairs pick the second part
@@ -365,7 +376,7 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   }
   if (eager) {
     // Same kernels without the graph, with events around the alignment phase.
-    enqueue_align(ctx, db, identity, mask, /*time_align=*/true, tally);
+    enqueue_align(ctx, db, identity, mask, /*time_align=*/true, tally, s);
   } else {
     hipGraphExec_t g2 = nullptr;
     for (const auto& e : db->gexec2)
@@ -375,20 +386,33 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
         for (auto& e : db->gexec2) (void)hipGraphExecDestroy(e.second);
         db->gexec2.clear();
       }
-      g2 = capture(s, [&] { enqueue_align(ctx, db, identity, mask, false); });
+      g2 = capture(s, [&] { enqueue_align(ctx, db, identity, mask, false, false, s); });
       db->gexec2.emplace_back(mask, g2);
     }
     NGIA_HIP(hipGraphLaunch(g2, s));
   }
   NGIA_HIP(hipEventRecord(ctx->ev[1], s));
-  Counters hc;
+  Counters hc, hh{};
   d2h(&hc, db->counters.p, sizeof(Counters), s);
-  uint64_t npairs = 0;
+  uint64_t npairs = 0, hpairs = 0;
   d2h(&npairs, db->off.as<uint64_t>() + n, 8, s);
+  if (head) {
+    NGIA_HIP(hipStreamWaitEvent(s, head->done, 0));
+    d2h(&hh, head->db->counters.p, sizeof(Counters), s);
+    d2h(&hpairs, head->db->off.as<uint64_t>() + head->db->n, 8, s);
+  }
   NGIA_HIP(hipStreamSynchronize(s));
   db->n_pairs = npairs;
   NGIA_HIP(hipEventRecord(ctx->ev[4], s));
-  finalize(ctx, db, hc.ecount);
+  if (hh.ecount) {
+    // the head's pairs (targets below db->t0) are disjoint from the main
+    // round's; their raw edges follow the main round's
+    NGIA_HIP(hipMemcpyAsync(db->ekeys.as<uint64_t>() + hc.ecount, head->db->ekeys.p, 8 * hh.ecount,
+                            hipMemcpyDeviceToDevice, s));
+    NGIA_HIP(hipMemcpyAsync(db->escores.as<uint32_t>() + hc.ecount, head->db->escores.p,
+                            4 * hh.ecount, hipMemcpyDeviceToDevice, s));
+  }
+  finalize(ctx, db, hc.ecount + hh.ecount);
   NGIA_HIP(hipEventRecord(ctx->ev[5], s));
   NGIA_HIP(hipEventSynchronize(ctx->ev[5]));
   ctx->timing.graph_ms = event_ms(ctx->ev[0], ctx->ev[1]);
@@ -406,10 +430,10 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
     db->bin_ms[b] = eager && launched ? event_ms(ctx->bin_ev[2 * b], ctx->bin_ev[2 * b + 1]) : 0.f;
   }
   uint64_t tiles = 0;
-  for (int b = 0; b < kBins; b++) tiles += hc.bin_tiles[b];
+  for (int b = 0; b < kBins; b++) tiles += hc.bin_tiles[b] + hh.bin_tiles[b];
   if (st) {
-    st->total_pairs = hc.total_cand;
-    st->distinct_pairs = npairs;
+    st->total_pairs = hc.total_cand + hh.total_cand;
+    st->distinct_pairs = npairs + hpairs;
     st->n_edges = db->n_edges;
     st->ref_tiles = tiles;
     st->ms_candidates = 0;
@@ -418,6 +442,163 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
     st->align_launches = launches;
     st->host_syncs = 3;
   }
+}
+
+// The head of a step: the leading run of records whose own band window (the
+// record against itself) is wider than 31 words at thr, in non-increasing
+// length order, at most kBucketSmallMax of them.  A pair (r, i) of the head
+// then has R >= T and a window at least that of (T, T): every head pair goes
+// to a wavefront bin (or, past the widest, the plain bin).
+uint32_t head_records(const ngia_db* db, float thr) {
+  uint32_t h = 0;
+  const uint32_t cap = std::min<uint32_t>(db->n, kBucketSmallMax);
+  while (h < cap) {
+    const uint32_t L = db->host_len[h];
+    if (h && L > db->host_len[h - 1]) break;
+    const Band b = band_edges(L, L, thr);
+    if (b.lsft / 32 + (b.rsft + 31) / 32 <= 31) break;
+    h++;
+  }
+  return h;
+}
+
+// Positions per wavefront of the head's chunked sketch.
+constexpr uint32_t kSketchChunk = 1024;
+// Fewer head records than this are not split off (NGIA_HEAD_MIN overrides
+// it, for tests on small sets): the split's own launches and waits cost about
+// 0.2 ms (50 head records of the 10,000-protein set at -i 50: 1.60 against
+// 1.35 ms per step).
+constexpr uint32_t kHeadMin = 256;
+
+// The head database of a step: records [0, h) of db, sharing its records,
+// codes and signatures (views), with rows, round workspace and the chunk
+// list of its sketch of its own.
+ngia_db* step_head(ngia_ctx* ctx, ngia_db* db, uint32_t h) {
+  if (!db->head || db->head->n != h) {
+    db->head = std::make_unique<ngia_db>();
+    std::vector<uint2> ch;
+    for (uint32_t i = 0; i < h; i++) {
+      const uint32_t L = db->host_len[i];
+      if (L == 0) continue;  // no window: its signature stays all ones
+      const uint32_t first = L >= 8 ? 7u : L - 1;
+      for (uint32_t p = 0; p < L; p += kSketchChunk)
+        ch.push_back(make_uint2(i, std::max(p, first)));
+    }
+    ngia_db* hd = db->head.get();
+    hd->sketch_chunks.ensure(8 * std::max<size_t>(ch.size(), 1));
+    if (!ch.empty())
+      NGIA_HIP(hipMemcpy(hd->sketch_chunks.p, ch.data(), 8 * ch.size(), hipMemcpyHostToDevice));
+    hd->n_sketch_chunks = (uint32_t)ch.size();
+  }
+  ngia_db* hd = db->head.get();
+  hd->ctx = ctx;
+  hd->n = h;
+  hd->n_words = db->n_words;
+  hd->max_len = *std::max_element(db->host_len.begin(), db->host_len.begin() + h);
+  hd->packed.view(db->packed.p, db->packed.bytes);
+  hd->word_off.view(db->word_off.p, 8ull * h);
+  hd->lengths.view(db->lengths.p, 4ull * h);
+  hd->codes.view(db->codes.p, db->codes.bytes);
+  hd->code_off.view(db->code_off.p, 8ull * h);
+  hd->sig.view(db->sig.p, 256ull * h);
+  hd->compact = db->compact;
+  hd->slot_code = db->slot_code;
+  if (db->compact)
+    hd->slot_codes.view(db->slot_codes.p, db->slot_codes.bytes);
+  else
+    hd->slot_codes.release();
+  hd->have_sig = true;
+  hd->rep_bands = NGIA_BANDS;
+  hd->t0 = 0;
+  return hd;
+}
+
+// Whether a head pair's window can exceed the widest wavefront kernel's:
+// with R >= T >= R * thr (shorter targets are length-rejected), the window
+// is at most lsft(R)/32 + ceil((R - R thr^2 + 32)/32), one word of slack.
+bool head_may_need_plain(const ngia_ctx* ctx, uint32_t max_len, float thr) {
+  const double R = max_len;
+  const double ws = std::ceil(R * (1.0 - thr) / 32.0) + std::ceil((R * (1.0 - (double)thr * thr) + 33.0) / 32.0) + 1.0;
+  const double K = ctx->wave_kmax;
+  return ws + K - 1.0 > 64.0 * K;
+}
+
+// Sketch, bucketing and the batched graph of the whole database, with the
+// head's graph beside the rest's sketch.  The pairs (r, i), r < i, of targets
+// i < h depend on the first h records alone (a record's representative in a
+// band precedes it in the stable (signature, index) order, so bucketing the
+// first h signatures gives the full bucketing's rows for them), and the main
+// round takes targets i >= h with the full rows: the two rounds' pairs are
+// disjoint and together every pair of the one-round graph.
+//
+// Side stream 1, with no host synchronisation: the head's signatures (one
+// wavefront per chunk of 1,024 positions: a few thousand long records, one
+// wavefront each, would leave most SIMDs idle), its rows (one block per
+// band), its candidates and its wavefront bins, longest first.  Context
+// stream, beside it: the rest's sketch, then after the head's signatures the
+// full bucketing, the main round from target h on, the head's raw edges, and
+// the sort.  The head's bins are latency bound (a few long pairs, one
+// wavefront each) and run on the SIMDs the VALU-bound sketch leaves between
+// its instructions.
+void graph_step(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands, bool split,
+                ngia_graph_stats* st) {
+  hipStream_t s = ctx->stream;
+  const uint32_t n = db->n;
+  const float thr = (float)identity / 100.0f;
+  db->sig.ensure(256ull * n);
+  db->rep.ensure(256ull * n);
+  db->bucket_scratch.ensure(bucket_scratch_bytes(n, ctx->bucket_wide));
+  uint32_t h = split ? head_records(db, thr) : 0u;
+  if (h < ctx->head_min || h >= n) h = 0;  // too few to pay for the split, or nothing beside them
+  const uint8_t* codes = db->codes.as<uint8_t>();
+  const uint64_t* coff = db->code_off.as<uint64_t>();
+  const uint32_t* len = db->lengths.as<uint32_t>();
+  uint32_t* sig = db->sig.as<uint32_t>();
+  NGIA_HIP(hipEventRecord(ctx->ev[6], s));
+  HeadRound hr;
+  if (h) {
+    ngia_db* hd = step_head(ctx, db, h);
+    hd->rep.ensure(256ull * h);
+    ensure_batched_space(hd, bands);
+    uint32_t mask = 0;
+    for (uint32_t k = 0; (1u << k) <= ctx->wave_kmax; k++) mask |= 1u << (kBinWave0 + k);
+    if (head_may_need_plain(ctx, hd->max_len, thr)) {
+      ensure_plain_bin_space(hd, bands);
+      mask |= 1u << kBinPlain;
+    }
+    // the rest's sketch first, so that the context stream is busy while the
+    // host enqueues the head's round
+    launch_sketch(codes, coff + h, len + h, n - h, sig + 64ull * h, s);
+    hipStream_t q = ctx->side[1];
+    NGIA_HIP(hipStreamWaitEvent(q, ctx->ev[6], 0));  // after the previous step's use
+    fill_u32(sig, 0xFFFFFFFFu, 64ull * h, q);
+    launch_sketch_chunks(codes, coff, len, hd->sketch_chunks.as<uint2>(), hd->n_sketch_chunks,
+                         kSketchChunk, sig, q);
+    NGIA_HIP(hipEventRecord(ctx->head_rows, q));
+    launch_bucket_small(sig, h, hd->rep.as<uint32_t>(), q);
+    enqueue_bins(ctx, hd, identity, bands, q);
+    enqueue_align(ctx, hd, identity, mask, false, false, q, /*serial=*/true);
+    NGIA_HIP(hipEventRecord(ctx->head_done, q));
+    hr.db = hd;
+    hr.done = ctx->head_done;
+    NGIA_HIP(hipStreamWaitEvent(s, ctx->head_rows, 0));
+  } else {
+    launch_sketch(codes, coff, len, n, sig, s);
+  }
+  launch_bucket(sig, n, db->rep.as<uint32_t>(), db->bucket_scratch.p, s, ctx->bucket_wide,
+                db->compact);
+  db->have_sig = true;
+  db->rep_bands = NGIA_BANDS;
+  NGIA_HIP(hipEventRecord(ctx->ev[7], s));
+  db->t0 = h;
+  try {
+    graph_batched(ctx, db, identity, bands, st, false, false, h ? &hr : nullptr);
+  } catch (...) {
+    db->t0 = 0;
+    throw;
+  }
+  db->t0 = 0;
+  ctx->timing.step_head = h;
 }
 
 }  // namespace
@@ -439,6 +620,11 @@ extern "C" int ngia_open(int device, ngia_ctx** out) {
     ctx->noalloc_scratch = noalloc_setup(device);
     const char* bw = std::getenv("NGIA_BUCKET_WIDE");
     ctx->bucket_wide = bw && bw[0] == '1';
+    const char* hm = std::getenv("NGIA_HEAD_MIN");
+    ctx->head_min = hm ? (uint32_t)std::strtoul(hm, nullptr, 10) : kHeadMin;
+    if (ctx->head_min < 1) ctx->head_min = 1;
+    const char* bs = std::getenv("NGIA_BUCKET_SMALL");
+    ctx->bucket_small = bs && bs[0] == '1';
     const char* sb = std::getenv("NGIA_SERIAL_BINS");
     ctx->serial_bins = sb && sb[0] == '1';
     const char* rl = std::getenv("NGIA_RAW_LCS");
@@ -451,6 +637,8 @@ extern "C" int ngia_open(int device, ngia_ctx** out) {
     NGIA_HIP(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
     for (auto& e : ctx->join) NGIA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& e : ctx->bin_ev) NGIA_HIP(hipEventCreate(&e));
+    NGIA_HIP(hipEventCreateWithFlags(&ctx->head_rows, hipEventDisableTiming));
+    NGIA_HIP(hipEventCreateWithFlags(&ctx->head_done, hipEventDisableTiming));
     return NGIA_OK;
   });
   if (rc != NGIA_OK) return rc;
@@ -623,6 +811,24 @@ extern "C" int ngia_db_graph(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint
   });
 }
 
+extern "C" int ngia_db_step(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
+                            uint32_t flags, ngia_graph_stats* st) {
+  if (!ctx || !db) return fail(ctx, NGIA_ERR_ARG, "null argument");
+  if (identity < 1 || identity > 99) return fail(ctx, NGIA_ERR_ARG, "identity should be 1-99");
+  if (bands < 1 || bands > NGIA_BANDS) return fail(ctx, NGIA_ERR_ARG, "bands should be 1-64");
+  if (db->n == 0) return fail(ctx, NGIA_ERR_ARG, "empty database");
+  if (flags & ~NGIA_STEP_NO_SPLIT) return fail(ctx, NGIA_ERR_ARG, "unknown step flags");
+  if (st) std::memset(st, 0, sizeof(*st));
+  return guarded(ctx, [&] {
+    auto t0 = std::chrono::steady_clock::now();
+    graph_step(ctx, db, identity, bands, !(flags & NGIA_STEP_NO_SPLIT), st);
+    ctx->timing.step_ms = event_ms(ctx->ev[6], ctx->ev[5]);
+    auto t1 = std::chrono::steady_clock::now();
+    if (st) st->ms_total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return NGIA_OK;
+  });
+}
+
 extern "C" int ngia_db_band_jobs(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t band,
                                  uint32_t* jobs, uint64_t cap, uint64_t* count) {
   if (!ctx || !db || !count) return fail(ctx, NGIA_ERR_ARG, "null argument");
@@ -768,8 +974,11 @@ extern "C" int ngia_bucket(ngia_ctx* ctx, const uint32_t* sig, uint32_t n, uint3
     drep.ensure(256ull * n);
     scratch.ensure(bucket_scratch_bytes(n, ctx->bucket_wide));
     h2d(dsig.p, sig, 256ull * n, ctx->stream);
-    launch_bucket(dsig.as<uint32_t>(), n, drep.as<uint32_t>(), scratch.p, ctx->stream,
-                  ctx->bucket_wide);
+    if (ctx->bucket_small && n <= kBucketSmallMax)
+      launch_bucket_small(dsig.as<uint32_t>(), n, drep.as<uint32_t>(), ctx->stream);
+    else
+      launch_bucket(dsig.as<uint32_t>(), n, drep.as<uint32_t>(), scratch.p, ctx->stream,
+                    ctx->bucket_wide);
     d2h(rep, drep.p, 256ull * n, ctx->stream);
     NGIA_HIP(hipStreamSynchronize(ctx->stream));
     return NGIA_OK;

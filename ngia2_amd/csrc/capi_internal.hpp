@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <memory>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -22,18 +23,28 @@ constexpr int kSideStreams = 4;
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool owned = true;  // false: a view of another buffer's memory
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    owned = true;
+  }
+  // A view of b bytes at ptr, which the buffer does not free.
+  void view(void* ptr, size_t b) {
+    release();
+    p = ptr;
+    bytes = b;
+    owned = false;
   }
   // Grow-only; contents are not preserved.
   void ensure(size_t b) {
     if (b <= bytes && p) return;
+    if (!owned) throw HipError{NGIA_ERR_ARG, "a view buffer cannot grow"};
     release();
     size_t want = b < 256 ? 256 : (b + 255) & ~(size_t)255;
     hipError_t e = hipMalloc(&p, want);
@@ -75,12 +86,19 @@ struct ngia_ctx {
   hipEvent_t fork = nullptr, join[ngia::kSideStreams] = {};
   // start/end of each bin's kernel in the eager batched form
   hipEvent_t bin_ev[2 * ngia::kBins] = {};
+  // a step's head: its rows ready, and its raw edges written
+  hipEvent_t head_rows = nullptr, head_done = nullptr;
   uint32_t wave_kmax = 8;  // largest k_align_wave<K> usable on this device
   // shapes of the window bins by size (autotuner): the 16-code table, and
   // databases of a compact alphabet
   ngia::RouteTable route{}, route_compact{};
   // NGIA_BUCKET_WIDE=1 selects the 38-bit-key bucketing that n >= 2^26 uses
   bool bucket_wide = false;
+  // NGIA_BUCKET_SMALL=1: ngia_bucket takes the one-block-per-band bucketing
+  // of a step's head whenever n <= kBucketSmallMax (a check of that path)
+  bool bucket_small = false;
+  // fewest head records that ngia_db_step splits off (env NGIA_HEAD_MIN)
+  uint32_t head_min = 256;
   // NGIA_SERIAL_BINS=1 runs the alignment bins one after another on the
   // context stream (each bin's kernel time is then its time alone)
   bool serial_bins = false;
@@ -103,6 +121,8 @@ struct ngia_ctx {
     for (auto& e : bin_ev)
       if (e) (void)hipEventDestroy(e);
     if (fork) (void)hipEventDestroy(fork);
+    if (head_rows) (void)hipEventDestroy(head_rows);
+    if (head_done) (void)hipEventDestroy(head_done);
     for (auto& q : side)
       if (q) (void)hipStreamDestroy(q);
     if (stream) (void)hipStreamDestroy(stream);
@@ -138,6 +158,19 @@ struct ngia_db {
   bool have_bins = false;
   ngia::Counters last{};
   float bin_ms[ngia::kBins] = {};
+  // record lengths (host copy), for the split of a step
+  std::vector<uint32_t> host_len;
+  // first target of the candidate pass: the rounds of a step split the
+  // targets at the head's end
+  uint32_t t0 = 0;
+  // the head of a step (ngia_db_step): records [0, head->n), with views of
+  // this database's records, codes and signatures and its own rows and
+  // round workspace
+  std::unique_ptr<ngia_db> head;
+  // (head only) its records' sketch in chunks of positions: (record, first
+  // position) per chunk
+  ngia::DevBuf sketch_chunks;
+  uint32_t n_sketch_chunks = 0;
   ~ngia_db() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     for (auto& e : gexec2) (void)hipGraphExecDestroy(e.second);
@@ -229,6 +262,8 @@ inline void db_upload(ngia_ctx* ctx, ngia_db* db, const uint32_t* packed, uint64
     lengths = len_v.data();
   }
   validate_records(packed, n_words, word_off, lengths, n, &db->max_len);
+  db->host_len.assign(lengths, lengths + n);
+  db->head.reset();
   db->n = n;
   db->n_words = n_words;
   hipStream_t s = ctx->stream;

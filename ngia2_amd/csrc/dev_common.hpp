@@ -88,6 +88,15 @@ __host__ __device__ inline uint32_t lcs_score(uint32_t lcs, uint32_t R, uint32_t
 size_t bucket_scratch_bytes(uint32_t n, bool force_wide = false);
 void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch,
                    hipStream_t stream, bool force_wide = false, bool few_values = false);
+// The same rows for n <= kBucketSmallMax records: one block per band, no
+// scratch and no host synchronisation.
+constexpr uint32_t kBucketSmallMax = 8192;
+void launch_bucket_small(const uint32_t* sig, uint32_t n, uint32_t* rep, hipStream_t stream);
+// Signatures of long records in chunks of `len` window positions:
+// chunks[c] = (record, first position); sig rows set to all ones beforehand.
+void launch_sketch_chunks(const uint8_t* codes, const uint64_t* code_off, const uint32_t* lengths,
+                          const uint2* chunks, uint32_t nchunks, uint32_t len, uint32_t* sig,
+                          hipStream_t stream);
 
 // Byte-per-residue codes (code of residue j at byte code_off[i] + j, padding
 // to a multiple of 32 is code 0) built from the packed planes.

@@ -232,7 +232,7 @@ __device__ inline void load_rep_tile(const uint32_t* __restrict__ rep, uint32_t 
 
 __global__ __launch_bounds__(256) void k_pair_count(const uint32_t* __restrict__ rep,
                                                     const uint32_t* __restrict__ len, uint32_t n,
-                                                    uint32_t bands, float thr,
+                                                    uint32_t t0, uint32_t bands, float thr,
                                                     uint32_t* __restrict__ cnt,
                                                     uint64_t* __restrict__ keep_out,
                                                     unsigned long long* __restrict__ total) {
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void k_pair_count(const uint32_t* __restrict__
   for (int tt = 0; tt < 16; tt++) {
     const uint32_t t = wave * 16 + tt, i = i0 + t;
     const uint32_t r = tile[lane][t];
-    const bool nd = lane < bands && i < n && r != i;
+    const bool nd = lane < bands && i < n && i >= t0 && r != i;
     lr[tt] = len[nd ? r : 0u];  // unconditional: rows hold indices < n
     li[tt] = len[i < n ? i : 0u];
     need |= (uint32_t)nd << tt;
@@ -309,10 +309,10 @@ __global__ __launch_bounds__(256) void k_pair_emit(const uint32_t* __restrict__ 
   }
 }
 
-void pair_count(const uint32_t* rep, const uint32_t* len, uint32_t n, uint32_t bands, float thr,
-                uint32_t* cnt, uint64_t* keep, unsigned long long* total, hipStream_t s) {
-  hipLaunchKernelGGL(k_pair_count, dim3((n + 63) / 64), dim3(256), 0, s, rep, len, n, bands, thr,
-                     cnt, keep, total);
+void pair_count(const uint32_t* rep, const uint32_t* len, uint32_t n, uint32_t t0, uint32_t bands,
+                float thr, uint32_t* cnt, uint64_t* keep, unsigned long long* total, hipStream_t s) {
+  hipLaunchKernelGGL(k_pair_count, dim3((n + 63) / 64), dim3(256), 0, s, rep, len, n, t0, bands,
+                     thr, cnt, keep, total);
   NGIA_HIP(hipGetLastError());
 }
 

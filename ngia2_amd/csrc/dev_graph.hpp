@@ -63,9 +63,9 @@ void fast_apply(const uint32_t* jobs, uint32_t count, uint32_t* parent, hipStrea
 // Candidates: the count pass writes cnt[i] (distinct pairs of target i),
 // keep[i] (the bands whose representatives are kept) and adds the count with
 // duplicates to *total; after cnt is scanned into off, the emit pass writes
-// the pairs grouped by target, in target order.
-void pair_count(const uint32_t* rep, const uint32_t* len, uint32_t n, uint32_t bands, float thr,
-                uint32_t* cnt, uint64_t* keep, unsigned long long* total, hipStream_t s);
+// the pairs grouped by target, in target order.  Targets below t0 get none.
+void pair_count(const uint32_t* rep, const uint32_t* len, uint32_t n, uint32_t t0, uint32_t bands,
+                float thr, uint32_t* cnt, uint64_t* keep, unsigned long long* total, hipStream_t s);
 void pair_emit(const uint32_t* rep, uint32_t n, uint32_t bands, const uint64_t* keep,
                const uint64_t* off, uint2* pairs, hipStream_t s);
 void bin_pairs(const uint2* pairs, const uint64_t* npairs, const uint32_t* len, float thr,

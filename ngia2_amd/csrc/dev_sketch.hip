@@ -20,27 +20,14 @@
 
 namespace ngia {
 
-__global__ __launch_bounds__(256) void k_sketch(const uint8_t* __restrict__ codes,
-                                                const uint64_t* __restrict__ code_off,
-                                                const uint32_t* __restrict__ lengths,
-                                                uint32_t n, uint32_t* __restrict__ sig) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t seq = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (seq >= n) return;
-  const uint32_t L = __builtin_amdgcn_readfirstlane(lengths[seq]);
-  // readfirstlane moves 32 bits: the 64-bit offset goes as two halves (code
-  // areas pass 2^31 bytes at a few million sequences).
-  const uint64_t co = code_off[seq];
-  const uint32_t co_lo = __builtin_amdgcn_readfirstlane((uint32_t)co);
-  const uint32_t co_hi = __builtin_amdgcn_readfirstlane((uint32_t)(co >> 32));
-  const uint32_t* cw = (const uint32_t*)(codes + (((uint64_t)co_hi << 32) | co_lo));
-  const uint32_t seed = lane * 0x9e3779b1u + 0x85ebca6bu;
-  const uint32_t s1 = seed ^ 61u ^ (seed >> 16);
-  uint32_t mn = 0xFFFFFFFFu;
-  uint32_t kmer = 0;
-  // Hashing starts at position 7, or at the last position of a read shorter
-  // than 8 (makedb.hpp:234-235).
-  const uint32_t first = L >= 8 ? 7u : (L ? L - 1 : 0u);
+// Hashes the windows ending at positions [j, end) of a record whose byte
+// codes start at cw (four per dword), the k-mer of the positions before j in
+// kmer; mn keeps the minimum.  Head to a 16-aligned position, then sixteen
+// positions per four-dword scalar load: one wait on the scalar cache per
+// sixteen windows.  The load reads whole 16-byte groups; a record's codes
+// are padded to 32 bytes, so it stays within the record's own padding.
+__device__ __forceinline__ void sketch_windows(const uint32_t* cw, uint32_t j, uint32_t end,
+                                               uint32_t kmer, uint32_t s1, uint32_t& mn) {
   auto hash = [&](uint32_t k1) {
     uint32_t h = k1 ^ s1;
     h = h + (h << 3);
@@ -51,22 +38,12 @@ __global__ __launch_bounds__(256) void k_sketch(const uint8_t* __restrict__ code
     h = h ^ (h >> 14);
     mn = h < mn ? h : mn;
   };
-  uint32_t j = 0;
-  // Prefix that only feeds the k-mer.
-  for (; j < first; j++) {
-    const uint32_t w = __builtin_amdgcn_readfirstlane(cw[j >> 2]);
-    kmer = (kmer << 4) + ((w >> (8 * (j & 3))) & 0xFu);
-  }
-  // Head up to a 16-aligned position, then sixteen positions per four-dword
-  // scalar load: one wait on the scalar cache per sixteen windows.  The load
-  // reads whole 16-byte groups; a record's codes are padded to 32 bytes, so
-  // it stays within the record's own padding.
-  for (; j < L && (j & 15); j++) {
+  for (; j < end && (j & 15); j++) {
     const uint32_t w = __builtin_amdgcn_readfirstlane(cw[j >> 2]);
     kmer = (kmer << 4) + ((w >> (8 * (j & 3))) & 0xFu);
     hash(kmer ^ (kmer >> 16));
   }
-  for (; j + 16 <= L; j += 16) {
+  for (; j + 16 <= end; j += 16) {
     const uint4 c4 = *(const uint4*)(cw + (j >> 2));
     const uint32_t ws[4] = {(uint32_t)__builtin_amdgcn_readfirstlane(c4.x),
                             (uint32_t)__builtin_amdgcn_readfirstlane(c4.y),
@@ -81,7 +58,7 @@ __global__ __launch_bounds__(256) void k_sketch(const uint8_t* __restrict__ code
       }
     }
   }
-  for (; j + 4 <= L; j += 4) {
+  for (; j + 4 <= end; j += 4) {
     const uint32_t w = __builtin_amdgcn_readfirstlane(cw[j >> 2]);
     #pragma unroll
     for (int m = 0; m < 4; m++) {
@@ -89,12 +66,94 @@ __global__ __launch_bounds__(256) void k_sketch(const uint8_t* __restrict__ code
       hash(kmer ^ (kmer >> 16));
     }
   }
-  for (; j < L; j++) {
+  for (; j < end; j++) {
     const uint32_t w = __builtin_amdgcn_readfirstlane(cw[j >> 2]);
     kmer = (kmer << 4) + ((w >> (8 * (j & 3))) & 0xFu);
     hash(kmer ^ (kmer >> 16));
   }
+}
+
+// The codes of record `seq` (the 64-bit offset moves to scalar registers as
+// two halves: code areas pass 2^31 bytes at a few million sequences).
+__device__ __forceinline__ const uint32_t* record_codes(const uint8_t* codes,
+                                                        const uint64_t* code_off, uint32_t seq) {
+  const uint64_t co = code_off[seq];
+  const uint32_t co_lo = __builtin_amdgcn_readfirstlane((uint32_t)co);
+  const uint32_t co_hi = __builtin_amdgcn_readfirstlane((uint32_t)(co >> 32));
+  return (const uint32_t*)(codes + (((uint64_t)co_hi << 32) | co_lo));
+}
+
+// Hashing starts at position 7, or at the last position of a read shorter
+// than 8 (makedb.hpp:234-235).
+__device__ __forceinline__ uint32_t first_window(uint32_t L) { return L >= 8 ? 7u : (L ? L - 1 : 0u); }
+
+// Lane k computes signature k: the per-lane half of the folded first two
+// hash steps, seed_k ^ 61 ^ (seed_k >> 16).
+__device__ __forceinline__ uint32_t lane_seed_term(uint32_t lane) {
+  const uint32_t seed = lane * 0x9e3779b1u + 0x85ebca6bu;
+  return seed ^ 61u ^ (seed >> 16);
+}
+
+__global__ __launch_bounds__(256) void k_sketch(const uint8_t* __restrict__ codes,
+                                                const uint64_t* __restrict__ code_off,
+                                                const uint32_t* __restrict__ lengths,
+                                                uint32_t n, uint32_t* __restrict__ sig) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t seq = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (seq >= n) return;
+  const uint32_t L = __builtin_amdgcn_readfirstlane(lengths[seq]);
+  const uint32_t* cw = record_codes(codes, code_off, seq);
+  const uint32_t first = first_window(L);
+  uint32_t kmer = 0;
+  for (uint32_t j = 0; j < first; j++) {  // the prefix only feeds the k-mer
+    const uint32_t w = __builtin_amdgcn_readfirstlane(cw[j >> 2]);
+    kmer = (kmer << 4) + ((w >> (8 * (j & 3))) & 0xFu);
+  }
+  uint32_t mn = 0xFFFFFFFFu;
+  sketch_windows(cw, first, L, kmer, lane_seed_term(lane), mn);
   sig[(uint64_t)seq * 64 + lane] = mn;
+}
+
+// Long records in pieces: one wavefront per (record, chunk of positions),
+// chunks[c] = (record, first window position); a chunk ends at the next
+// multiple of `len` or the record's end; the k-mer is rebuilt from the seven codes
+// before the chunk (codes before position 0 count as 0, as in k_sketch).
+// The minimum over a record's chunks is the same whatever their order:
+// each wavefront folds its chunk's minima into sig with atomicMin, over
+// signatures set to all ones first.
+__global__ __launch_bounds__(256) void k_sketch_chunks(const uint8_t* __restrict__ codes,
+                                                       const uint64_t* __restrict__ code_off,
+                                                       const uint32_t* __restrict__ lengths,
+                                                       const uint2* __restrict__ chunks,
+                                                       uint32_t nchunks, uint32_t len,
+                                                       uint32_t* __restrict__ sig) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t c = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (c >= nchunks) return;
+  const uint2 ch = chunks[c];
+  const uint32_t seq = __builtin_amdgcn_readfirstlane(ch.x);
+  const uint32_t p0 = __builtin_amdgcn_readfirstlane(ch.y);
+  const uint32_t L = __builtin_amdgcn_readfirstlane(lengths[seq]);
+  const uint32_t* cw = record_codes(codes, code_off, seq);
+  const uint32_t stop = p0 - p0 % len + len;
+  const uint32_t end = stop < L ? stop : L;
+  uint32_t kmer = 0;
+  for (uint32_t j = p0 >= 7 ? p0 - 7 : 0u; j < p0; j++) {
+    const uint32_t w = __builtin_amdgcn_readfirstlane(cw[j >> 2]);
+    kmer = (kmer << 4) + ((w >> (8 * (j & 3))) & 0xFu);
+  }
+  uint32_t mn = 0xFFFFFFFFu;
+  sketch_windows(cw, p0, end, kmer, lane_seed_term(lane), mn);
+  atomicMin(&sig[(uint64_t)seq * 64 + lane], mn);
+}
+
+void launch_sketch_chunks(const uint8_t* codes, const uint64_t* code_off, const uint32_t* lengths,
+                          const uint2* chunks, uint32_t nchunks, uint32_t len, uint32_t* sig,
+                          hipStream_t stream) {
+  if (nchunks == 0) return;
+  hipLaunchKernelGGL(k_sketch_chunks, dim3((nchunks + 3) / 4), dim3(256), 0, stream, codes,
+                     code_off, lengths, chunks, nchunks, len, sig);
+  NGIA_HIP(hipGetLastError());
 }
 
 void launch_sketch(const uint8_t* codes, const uint64_t* code_off, const uint32_t* lengths,
@@ -326,6 +385,57 @@ __global__ __launch_bounds__(BS) void k_part_local(const uint64_t* __restrict__ 
     part_sort_tier<BS, I3>(vals + s, cnt, pb, row, rep, storage, lastk, lastv);
   else if (threadIdx.x == 0)
     atomicAdd(overflow, 1u);
+}
+
+// Bucketing of at most kBucketSmallMax records, for the head of a step: one
+// block per band sorts the band's n (mix(sig), i) pairs in LDS, stably, and
+// points each element equal to its predecessor at that predecessor, as
+// k_part_local does for a partition.  No global sort, no look-back between
+// blocks and no host synchronisation, so it runs beside other kernels.
+template <int BS, int ITEMS>
+__global__ __launch_bounds__(BS) void k_bucket_small(const uint32_t* __restrict__ sig, uint32_t n,
+                                                     uint32_t* __restrict__ rep) {
+  using Sort = rocprim::block_radix_sort<uint32_t, BS, ITEMS, uint32_t>;
+  __shared__ typename Sort::storage_type storage;
+  __shared__ uint32_t lastk[BS], lastv[BS];
+  const uint32_t b = blockIdx.x, t = threadIdx.x;
+  const uint64_t row = (uint64_t)b * n;
+  uint32_t key[ITEMS], val[ITEMS];
+  #pragma unroll
+  for (int j = 0; j < ITEMS; j++) {
+    const uint32_t q = t * ITEMS + j;  // blocked: input order = ascending i
+    if (q < n) {
+      key[j] = sig_mix(sig[(uint64_t)q * 64 + b]);
+      val[j] = q;
+      rep[row + q] = q;  // a bucket head keeps its own index
+    } else {  // padding sorts after every element (stable)
+      key[j] = 0xFFFFFFFFu;
+      val[j] = 0xFFFFFFFFu;
+    }
+  }
+  __syncthreads();
+  Sort().sort(key, val, storage, 0, 32);
+  lastk[t] = key[ITEMS - 1];
+  lastv[t] = val[ITEMS - 1];
+  __syncthreads();
+  uint32_t pk = t ? lastk[t - 1] : 0u, pv = t ? lastv[t - 1] : 0u;
+  #pragma unroll
+  for (int j = 0; j < ITEMS; j++) {
+    const uint32_t q = t * ITEMS + j;
+    if (q >= 1 && q < n && key[j] == pk) rep[row + val[j]] = pv;
+    pk = key[j];
+    pv = val[j];
+  }
+}
+
+void launch_bucket_small(const uint32_t* sig, uint32_t n, uint32_t* rep, hipStream_t stream) {
+  if (n == 0) return;
+  if (n > kBucketSmallMax) throw HipError{NGIA_ERR_ARG, "too many records for the small bucketing"};
+  if (n <= 2048)
+    hipLaunchKernelGGL((k_bucket_small<256, 8>), dim3(64), dim3(256), 0, stream, sig, n, rep);
+  else
+    hipLaunchKernelGGL((k_bucket_small<1024, 8>), dim3(64), dim3(1024), 0, stream, sig, n, rep);
+  NGIA_HIP(hipGetLastError());
 }
 
 namespace {

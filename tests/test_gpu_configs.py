@@ -14,6 +14,8 @@ finishes in seconds.
   oracle's assignment.
 Each test prints the case before touching the GPU.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -151,6 +153,53 @@ def test_config4_graph(ctx, config4, identity):
     want, bins = check_graph(ctx, d, rep, identity)
     assert len(want) > 500
     assert any(k == "wave" for k, _ in bins) and any(k == "window" for k, _ in bins), bins
+
+
+@pytest.fixture(scope="module")
+def split_ctx():
+    """A context that splits heads of 16 records or more (the samples here
+    hold a few dozen to a few hundred head records)."""
+    os.environ["NGIA_HEAD_MIN"] = "16"
+    try:
+        c = ng.Context(0)
+    finally:
+        del os.environ["NGIA_HEAD_MIN"]
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("cfg,identity", [(4, 90), (4, 70), (2, 50), (1, 90)])
+def test_step_split_equals_stages(split_ctx, config2, config4, config1, cfg, identity):
+    """ngia_db_step: the head's round (records whose own window exceeds 31
+    words) beside the rest's sketch, then the main round from the head's end.
+    Signatures, rows and edges equal the oracle's and those of the three
+    stages run one after another; a second step (replayed main round) agrees,
+    and a plain graph() afterwards still covers every target."""
+    if cfg == 1:
+        d = config1[2]
+        p = ng.PackedDB()
+        p.packed, p.word_off, p.lengths, p.n = d["packed"], d["word_off"], d["lengths"], d["n"]
+        d, sig, rep = p, orc.sketch_all(p.packed, p.word_off), d["rep"]
+    else:
+        d, sig, rep = config4 if cfg == 4 else config2
+    want, tot = orc.build_graph(d.packed, d.word_off, d.lengths, rep, 64, identity)
+    say("step, config", cfg, "-i", identity, "records", d.n)
+    ctx = split_ctx
+    db = ng.Database(ctx, d.packed, d.word_off, d.lengths)
+    for split in (True, False, True):
+        st = db.step(identity, 64, split=split)
+        head = ctx.timing()["step_head"]
+        if split and cfg != 1:
+            assert head > 0
+        if not split or cfg == 1:
+            assert head == 0
+        np.testing.assert_array_equal(db.edges(), want)
+        assert st["total_pairs"] == tot
+        np.testing.assert_array_equal(db.sigs(), sig)
+        np.testing.assert_array_equal(db.reps(), rep)
+    db.graph(identity, 64, ng.GRAPH_BATCHED)
+    np.testing.assert_array_equal(db.edges(), want)
+    db.close()
 
 
 def test_config5_serving(ctx):

@@ -93,6 +93,26 @@ def test_bucket_wide_keys(c1db):
     np.testing.assert_array_equal(got, d["rep"])
 
 
+def test_bucket_small_blocks():
+    """The head's bucketing (one block per band, the sort in LDS), through
+    NGIA_BUCKET_SMALL: both block sizes and the capacity, 1 to 8,192 records,
+    with heavy collisions and all-ones signatures (the sort's padding key)."""
+    rng = np.random.default_rng(17)
+    os.environ["NGIA_BUCKET_SMALL"] = "1"
+    try:
+        c2 = ng.Context(0)
+    finally:
+        del os.environ["NGIA_BUCKET_SMALL"]
+    with c2:
+        for n in (1, 2, 100, 2048, 2049, 5000, 8192):
+            say("bucket small", n)
+            sig = rng.integers(0, 1 << 32, size=(n, 64), dtype=np.uint64).astype(np.uint32)
+            sig[:, 1] = rng.integers(0, 7, size=n)
+            sig[:, 2] = 0xFFFFFFFF
+            sig[::3, 3] = 0xFFFFFFFF
+            np.testing.assert_array_equal(c2.bucket(sig), orc.bucket(sig))
+
+
 def test_bucket_collisions(ctx):
     # heavy signature collisions: few distinct values per band
     rng = np.random.default_rng(5)
