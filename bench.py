@@ -102,12 +102,12 @@ def copy_probe(nbytes):
     log("copy probe: %d bytes device to device" % nbytes)
 
 
-def measure(ng, np, ctx, info, d, identity, steps, warmup, plain_steps, label):
+def measure(ng, np, ctx, info, d, identity, steps, warmup, plain_steps, label, split=True):
     db = ng.Database(ctx, d.packed, d.word_off, d.lengths)
 
     def step(mode):
         if mode == "step":  # sketch + bucketing + graph in one call, head round split off
-            st = db.step(identity, 64, split=True)
+            st = db.step(identity, 64, split=split)
             tm = ctx.timing()
             return {"step_ms": tm["step_ms"], "head": tm["step_head"], "stats": st}
         db.sketch()
@@ -472,6 +472,9 @@ def main():
     ap.add_argument("--serve-queries", type=int, default=300000,
                     help="queries of the serving stream over the resident set (0: skip)")
     ap.add_argument("--no-tune", action="store_true", help="skip the window-bin autotuner")
+    ap.add_argument("--no-split", action="store_true",
+                    help="time the step without the head round (every dispatch of a kernel then "
+                         "does the same work: for counter profiles)")
     ap.add_argument("--no-e2e", action="store_true", help="skip the command-line end-to-end run")
     ap.add_argument("--chunks", type=int, default=4,
                     help="blocks of the out-of-HBM graph run (0 skips it)")
@@ -524,7 +527,7 @@ def main():
                                                         time.perf_counter() - t0))
     plain_steps = 0 if args.no_plain else args.plain_steps
     m = measure(ng, np, ctx, info, d, args.identity, args.steps, args.warmup, plain_steps,
-                "config %d" % args.config)
+                "config %d" % args.config, split=not args.no_split)
     st = m["stats"]
     workload = ("%d proteins in %d families, %s lengths (median %g, max %d), -i %d, 64 bands"
                 % (d.n, params.n_families,
@@ -601,7 +604,7 @@ def main():
                                eager_ms_per_step=round(m["eager_ms_per_step"], 3),
                                eager_edges_identical=m["eager_edges_identical"],
                                tally_edges_identical=m["tally_edges_identical"]),
-           "align_kernels": align_rows[:8],
+           "align_kernels": align_rows,
            "autotune": tune,
            "device": {"name": info["name"], "arch": info["arch"], "cu": info["cu_count"],
                       "clock_mhz": info["clock_khz"] / 1e3}}
@@ -638,7 +641,7 @@ def main():
         for ident in (int(v) for v in idents.split("/")):
             plain = 0 if args.no_plain else (3 if cfg != 3 else 1)
             m2 = measure(ng, np, ctx, info, d2, ident, args.steps, args.warmup, plain,
-                         "config %d -i %d" % (cfg, ident))
+                         "config %d -i %d" % (cfg, ident), split=not args.no_split)
             rows2, phase2 = align_rooflines(m2, peak)
             r = {"identity": ident, "value": round(m2["value"], 1),
                  "ms_per_step": round(m2["ms_per_step"], 3),
