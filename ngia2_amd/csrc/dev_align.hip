@@ -545,19 +545,29 @@ __global__ __launch_bounds__(64) void k_align_win(AlignArgs A, const uint2* __re
           const int32_t a = (int32_t)(bd.lsft / 32);
           const int32_t nw = (int32_t)((R + 31) / 32);
           const uint32_t* tc = (const uint32_t*)(rcodes + A.code_off[pr.y]);
-          // Window at block 0: relative word j <-> absolute j - a, register
-          // j, slot j.  Words outside the record get the masks of all-zero
-          // planes: 0 for every nonzero code.
+          // The first block's codes, and the planes of the W window words
+          // at block 0, all in one round trip: relative word j <-> absolute
+          // j - a, register j, slot j.  A word outside the record reads the
+          // packed area's first four words (the buffer holds at least 64) and
+          // is zeroed (the masks of all-zero planes: 0 for every nonzero
+          // code).  Loaded word by word, before their masks were stored, the
+          // planes took W round trips.
+          st.nc0 = *(const uint4*)tc;
+          st.nc1 = *(const uint4*)(tc + 4);
+          uint4 pw[W];
+          #pragma unroll
+          for (int j = 0; j < W; j++) {
+            const int32_t w = j - a;
+            const uint32_t* pl = (w >= 0 && w < nw) ? rr + 1 + 4 * w : A.packed;
+            pw[j] = make_uint4(pl[0], pl[1], pl[2], pl[3]);
+          }
           #pragma unroll
           for (int j = 0; j < W; j++) {
             rows[j] = 0xFFFFFFFFu;
             const int32_t w = j - a;
-            uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
-            if (w >= 0 && w < nw) {
-              const uint32_t* pl = rr + 1 + 4 * w;
-              p0 = pl[0]; p1 = pl[1]; p2 = pl[2]; p3 = pl[3];
-            }
-            win_store_masks<NC>(lds + j * kSlot, lane, p0, p1, p2, p3, A.slot_code);
+            const bool in = w >= 0 && w < nw;
+            win_store_masks<NC>(lds + j * kSlot, lane, in ? pw[j].x : 0u, in ? pw[j].y : 0u,
+                                in ? pw[j].z : 0u, in ? pw[j].w : 0u, A.slot_code);
           }
           st.lcs = 0u;
           st.rr = rr;
@@ -565,8 +575,6 @@ __global__ __launch_bounds__(64) void k_align_win(AlignArgs A, const uint2* __re
           st.c = (int32_t)((bd.rsft + 31) / 32);
           st.nw = nw;
           st.nb = (T + 31) / 32;
-          st.nc0 = *(const uint4*)tc;
-          st.nc1 = *(const uint4*)(tc + 4);
           cb0 = 0;
           active = true;
         }
