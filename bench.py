@@ -581,7 +581,7 @@ def main():
                                      "indels": [params.indel_lo, params.indel_hi]},
                       "form": "one ngia_db_step per step: sketch, bucketing and the batched "
                               "graph (length-binned kernels, captured hipGraph), with the head "
-                              "round (records whose own band window exceeds 31 words) beside "
+                              "round (records whose own band window spans 64 words or more) beside "
                               "the rest's sketch"},
            "value_covers": "sequences per second through the device hot path: sketch, band "
                            "bucketing and the graph up to the sorted unique edge list on the "

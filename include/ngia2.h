@@ -199,9 +199,9 @@ int ngia_db_graph(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
  * the signatures, rows and sorted unique edges of ngia_db_sketch,
  * ngia_db_bucket and ngia_db_graph(NGIA_GRAPH_BATCHED) in sequence.  No
  * reference counterpart (the reference runs makeDB and clustering as two
- * programs, main.cu:62-117).  The records whose own band window exceeds 31
- * words at this identity (the leading run: makeDB orders records longest
- * first) form a head; its graph, whose pairs depend on the head alone,
+ * programs, main.cu:62-117).  The records whose own band window spans at
+ * least 64 words at this identity (the leading run: makeDB orders records
+ * longest first) form a head; its graph, whose pairs depend on the head alone,
  * runs beside the sketch of the rest, and the main round takes the other
  * targets.  NGIA_STEP_NO_SPLIT runs the three stages one after another.
  * Bin figures (ngia_db_bin_stats) and pair scores then cover the main

@@ -275,7 +275,7 @@ class Database:
         """Sketch, bucketing and the batched graph in one call (ngia_db_step):
         the same signatures, rows and edges as sketch(), bucket() and
         graph(identity, bands); with split, the head's graph (records whose
-        own band window exceeds 31 words) runs beside the rest's sketch."""
+        own band window spans 64 words or more) runs beside the rest's sketch."""
         st = _lib.GraphStats()
         _check(_lib.lib().ngia_db_step(self.ctx.h, self.h, identity, bands,
                                        0 if split else _lib.STEP_NO_SPLIT, C.byref(st)),

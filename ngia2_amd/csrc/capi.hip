@@ -445,18 +445,22 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
 }
 
 // The head of a step: the leading run of records whose own band window (the
-// record against itself) is wider than 31 words at thr, in non-increasing
-// length order, at most kBucketSmallMax of them.  A pair (r, i) of the head
-// then has R >= T and a window at least that of (T, T): every head pair goes
-// to a wavefront bin (or, past the widest, the plain bin).
-uint32_t head_records(const ngia_db* db, float thr) {
+// record against itself) is at least min_window words (64 by default, at
+// least 32) at thr, in non-increasing length order, at most kBucketSmallMax
+// of them.  A pair (r, i) of the head then has R >= T and a window at least
+// that of (T, T): every head pair goes to a wavefront bin (or, past the
+// widest, the plain bin).  With 64 the head holds the pairs of K >= 2, the
+// long latency-bound ones; with 32 (the K = 1 pairs too) the heavy-tailed
+// step measured the same, and the log-normal set's 8,192-record head at
+// -i 50 lost 1%; with 128, 11.6 against 11.3 ms on the heavy-tailed set.
+uint32_t head_records(const ngia_db* db, float thr, uint32_t min_window) {
   uint32_t h = 0;
   const uint32_t cap = std::min<uint32_t>(db->n, kBucketSmallMax);
   while (h < cap) {
     const uint32_t L = db->host_len[h];
     if (h && L > db->host_len[h - 1]) break;
     const Band b = band_edges(L, L, thr);
-    if (b.lsft / 32 + (b.rsft + 31) / 32 <= 31) break;
+    if (b.lsft / 32 + (b.rsft + 31) / 32 < min_window) break;
     h++;
   }
   return h;
@@ -548,7 +552,7 @@ void graph_step(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands, b
   db->sig.ensure(256ull * n);
   db->rep.ensure(256ull * n);
   db->bucket_scratch.ensure(bucket_scratch_bytes(n, ctx->bucket_wide));
-  uint32_t h = split ? head_records(db, thr) : 0u;
+  uint32_t h = split ? head_records(db, thr, ctx->head_window) : 0u;
   if (h < ctx->head_min || h >= n) h = 0;  // too few to pay for the split, or nothing beside them
   const uint8_t* codes = db->codes.as<uint8_t>();
   const uint64_t* coff = db->code_off.as<uint64_t>();
@@ -620,6 +624,8 @@ extern "C" int ngia_open(int device, ngia_ctx** out) {
     ctx->noalloc_scratch = noalloc_setup(device);
     const char* bw = std::getenv("NGIA_BUCKET_WIDE");
     ctx->bucket_wide = bw && bw[0] == '1';
+    const char* hw = std::getenv("NGIA_HEAD_WINDOW");
+    ctx->head_window = hw ? std::max<uint32_t>(32u, (uint32_t)std::strtoul(hw, nullptr, 10)) : 64u;
     const char* hm = std::getenv("NGIA_HEAD_MIN");
     ctx->head_min = hm ? (uint32_t)std::strtoul(hm, nullptr, 10) : kHeadMin;
     if (ctx->head_min < 1) ctx->head_min = 1;

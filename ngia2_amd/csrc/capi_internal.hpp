@@ -97,8 +97,11 @@ struct ngia_ctx {
   // NGIA_BUCKET_SMALL=1: ngia_bucket takes the one-block-per-band bucketing
   // of a step's head whenever n <= kBucketSmallMax (a check of that path)
   bool bucket_small = false;
-  // fewest head records that ngia_db_step splits off (env NGIA_HEAD_MIN)
+  // fewest head records that ngia_db_step splits off (env NGIA_HEAD_MIN),
+  // and the narrowest own band window of a head record, in words (env
+  // NGIA_HEAD_WINDOW, at least 32: every head pair must go to a wavefront bin)
   uint32_t head_min = 256;
+  uint32_t head_window = 64;
   // NGIA_SERIAL_BINS=1 runs the alignment bins one after another on the
   // context stream (each bin's kernel time is then its time alone)
   bool serial_bins = false;

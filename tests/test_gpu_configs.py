@@ -157,13 +157,16 @@ def test_config4_graph(ctx, config4, identity):
 
 @pytest.fixture(scope="module")
 def split_ctx():
-    """A context that splits heads of 16 records or more (the samples here
-    hold a few dozen to a few hundred head records)."""
+    """A context that splits heads of 16 records or more, whose own band
+    windows span 32 words or more (the samples here hold a few dozen to a few
+    hundred such records)."""
     os.environ["NGIA_HEAD_MIN"] = "16"
+    os.environ["NGIA_HEAD_WINDOW"] = "32"
     try:
         c = ng.Context(0)
     finally:
         del os.environ["NGIA_HEAD_MIN"]
+        del os.environ["NGIA_HEAD_WINDOW"]
     yield c
     c.close()
 
