@@ -7,7 +7,7 @@
 // <32>, four and two pairs per wavefront with one word per lane; and
 // k_align_wave<1>, a wavefront per pair.  For each W the tuner times every
 // shape on synthetic pairs whose band window is exactly W (equal-length
-// random sequences) at bin sizes of 16 to 65,536 pairs, and installs the
+// random sequences) at bin sizes of 16 to 262,144 pairs, and installs the
 // fastest shape per (W, size class); k_bin_offsets then moves each window
 // bin's pairs to the bin of its shape on the device.
 #include <random>
@@ -18,11 +18,11 @@ using namespace ngia;
 
 namespace {
 
-constexpr uint32_t kTuneCounts[] = {16, 64, 256, 1024, 4096, 16384, 65536};
-constexpr int kNumCounts = 7;
+constexpr uint32_t kTuneCounts[] = {16, 64, 256, 1024, 4096, 16384, 65536, 262144};
+constexpr int kNumCounts = 8;
 static_assert(kNumCounts <= kTuneClasses, "size classes");
 constexpr uint32_t kTunePairs = 4096;  // distinct synthetic pairs; larger bins repeat them
-constexpr uint32_t kTuneList = 65536;
+constexpr uint32_t kTuneList = 262144;
 
 // Band window of a pair of lengths R >= T.
 uint32_t window_of(uint32_t R, uint32_t T, float thr) {

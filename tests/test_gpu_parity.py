@@ -732,13 +732,14 @@ def test_autotune_installs_fastest_shapes(ctx):
     say("autotune")
     pts = ctx.autotune(90)
     t = ctx.tuning()
-    assert t["class_size"][:7] == [16, 64, 256, 1024, 4096, 16384, 65536]
+    sizes = [16, 64, 256, 1024, 4096, 16384, 65536, 262144]
+    assert t["class_size"] == sizes
     widths = sorted({p["width"] for p in pts if not p["dna"]})
     assert widths == list(range(3, 17))  # W = 2 cannot pass the length test
     for p in pts:
         assert set(p["ms"]) == ({"window", "seg32", "wave"} | ({"seg16"} if p["width"] <= 15 else set()))
         assert all(v > 0 for v in p["ms"].values())
-        k = [16, 64, 256, 1024, 4096, 16384, 65536].index(p["pairs"])
+        k = sizes.index(p["pairs"])
         table = t["shape_compact"] if p["dna"] else t["shape"]
         best = min(p["ms"], key=lambda s_: (p["ms"][s_], ng.SHAPES.index(s_)))
         assert table[p["width"]][k] == ng.SHAPES.index(best), p
