@@ -206,7 +206,7 @@ void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
     if ((1u << k) > ctx->wave_kmax || !holds(bin)) return;
     mark(bin, 0, q);
     launch_align_wave(1 << k, args(bin), db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[bin],
-                      &dc->bin_cnt[bin], &dc->work[bin], db->score.as<uint32_t>(),
+                      &dc->bin_cnt[bin], dc->queue[bin], db->score.as<uint32_t>(),
                       persistent_blocks(ctx, wave_lds_bytes(1 << k)), q);
     mark(bin, 1, q);
   };
@@ -222,7 +222,7 @@ void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
     if (!holds(bin)) return;
     mark(bin, 0, q);
     launch_align_seg(bin == kBinSeg16 ? 16 : 32, args(bin), db->pairs.as<uint2>(), db->list.as<uint32_t>(),
-                     &dc->bin_off[bin], &dc->bin_cnt[bin], &dc->work[bin], db->score.as<uint32_t>(),
+                     &dc->bin_off[bin], &dc->bin_cnt[bin], dc->queue[bin], db->score.as<uint32_t>(),
                      ctx->prop.multiProcessorCount * seg_blocks_per_cu(bin == kBinSeg16 ? 16 : 32), q);
     mark(bin, 1, q);
   };
@@ -235,7 +235,7 @@ void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
     hipStream_t q = fork ? ctx->side[kWinStreams[slot++ % 3]] : s;
     mark(W, 0, q);
     launch_align_win(W, args(W), db->pairs.as<uint2>(), db->list.as<uint32_t>(), &dc->bin_off[W],
-                     &dc->bin_cnt[W], &dc->work[W], db->score.as<uint32_t>(),
+                     &dc->bin_cnt[W], dc->queue[W], db->score.as<uint32_t>(),
                      persistent_blocks(ctx, win_lds_bytes(W, a0.slots != nullptr)), q);
     mark(W, 1, q);
   }
@@ -393,12 +393,12 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   }
   NGIA_HIP(hipEventRecord(ctx->ev[1], s));
   Counters hc, hh{};
-  d2h(&hc, db->counters.p, sizeof(Counters), s);
+  d2h(&hc, db->counters.p, kCounterStats, s);
   uint64_t npairs = 0, hpairs = 0;
   d2h(&npairs, db->off.as<uint64_t>() + n, 8, s);
   if (head) {
     NGIA_HIP(hipStreamWaitEvent(s, head->done, 0));
-    d2h(&hh, head->db->counters.p, sizeof(Counters), s);
+    d2h(&hh, head->db->counters.p, kCounterStats, s);
     d2h(&hpairs, head->db->off.as<uint64_t>() + head->db->n, 8, s);
   }
   NGIA_HIP(hipStreamSynchronize(s));

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <functional>
 #include <memory>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -69,9 +70,12 @@ struct Counters {
   uint32_t bin_cnt[kBins];
   uint32_t bin_off[kBins + 1];
   uint32_t cursor[kBins];
-  uint32_t work[kBins];
   uint32_t remap[kBins];
+  // the work queues of the bins' kernels; last, so that the statistics the
+  // host reads (kCounterStats bytes) stay small
+  alignas(256) uint32_t queue[kBins][kQueueWords];
 };
+constexpr size_t kCounterStats = offsetof(Counters, queue);
 
 }  // namespace ngia
 

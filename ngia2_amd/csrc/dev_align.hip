@@ -313,35 +313,68 @@ __device__ inline bool cannot_pass(uint32_t lcs_now, uint32_t done, uint32_t R, 
 }
 
 // A wavefront's claim on its bin's work queue.  Every pair taken through a
-// per-pair atomic on the bin's one counter serialises a bin of many short
-// pairs on that address, so one atomic claims `chunk` consecutive items and
-// the wave hands them out in order.  The chunk follows the bin's size, so
-// each wave still gets many claims and the queue drains evenly:
-// count / (16 waves' worth), between `lo` and `hi` (lo >= the most items
-// handed out at once).  next/end are wave-uniform; take() is called by the
-// whole wave at a convergent point.
+// per-pair atomic on one counter serialises a bin of many short pairs on that
+// address, so one atomic claims `chunk` consecutive items of a slice and the
+// wave hands them out in order.  The chunk follows the bin's size, so each
+// wave still gets many claims and the queue drains evenly: count / (16
+// waves' worth), between `lo` and `hi` (lo >= the most items handed out at
+// once).  The wave starts on the slice of its XCD (kQueueSlices slices, one
+// counter each); when a claim comes back empty it reads every slice's
+// counter and moves to the next slice with items left, and with none left
+// the queue is drained.  All members are wave-uniform; take() is called by
+// the whole wave at a convergent point.
 struct WaveClaim {
-  uint32_t next = 0, end = 0, chunk = 1;
-  __device__ WaveClaim(uint32_t count, uint32_t lo, uint32_t hi) {
+  uint32_t next = 0, end = 0, chunk = 1, count, slice;
+  bool drained = false;
+  __device__ WaveClaim(uint32_t count_, uint32_t lo, uint32_t hi) : count(count_) {
     const uint32_t per = count / (gridDim.x * (blockDim.x / 64u) * 16u);
     chunk = per < lo ? lo : per > hi ? hi : per;
+    slice = blockIdx.x % kQueueSlices;
+  }
+  __device__ uint32_t slice_lo(uint32_t s) const {
+    return (uint32_t)((uint64_t)count * s / kQueueSlices);
   }
   // n (wave-uniform, n <= lo) lanes take an item each; `rank` (< n) is this
-  // lane's place among them.  Items from the counter's `count` on mean the
-  // queue is drained (every item below it is handed out exactly once: a wave
-  // hands out its claims in order and leaves only past `count`).
-  __device__ uint32_t take(uint32_t n, uint32_t rank, uint32_t* work) {
-    const uint32_t avail = end - next;
-    uint32_t item = next + rank;
-    if (n > avail) {
-      uint32_t b = 0;
-      if ((threadIdx.x & 63u) == 0) b = atomicAdd(work, chunk);
-      b = __builtin_amdgcn_readfirstlane(b);
-      if (rank >= avail) item = b + (rank - avail);
-      next = b + (n - avail);
-      end = b + chunk;
-    } else {
-      next += n;
+  // lane's place among them.  A lane that gets no item (the queue is
+  // drained) gets 0xFFFFFFFF, which is >= count.  Every item below count is
+  // handed out exactly once: each slice's counter hands out its items once,
+  // and a wave hands out its claims in order.
+  __device__ uint32_t take(uint32_t n, uint32_t rank, uint32_t* queue) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t item = 0xFFFFFFFFu, given = 0;
+    while (given < n && !drained) {
+      if (next == end) {
+        const uint32_t lo = slice_lo(slice), len = slice_lo(slice + 1) - lo;
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(queue + slice * kQueueStride, chunk);
+        b = __builtin_amdgcn_readfirstlane(b);
+        if (b < len) {
+          next = lo + b;
+          end = lo + (len - b < chunk ? len : b + chunk);
+        } else {
+          // this slice is exhausted: find one with items left, looking from
+          // the next slice on (a counter only grows, so an exhausted slice
+          // stays so)
+          bool left = false;
+          if (lane < kQueueSlices) {
+            const uint32_t c = __hip_atomic_load(queue + lane * kQueueStride, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            left = c < slice_lo(lane + 1) - slice_lo(lane);
+          }
+          const uint32_t m = (uint32_t)__builtin_amdgcn_ballot_w64(left);
+          if (m == 0) {
+            drained = true;
+          } else {
+            const uint32_t rot = ((m >> slice) | (m << (kQueueSlices - slice))) & 0xFFu;
+            slice = (slice + (uint32_t)__builtin_ctz(rot)) % kQueueSlices;
+          }
+        }
+        continue;
+      }
+      const uint32_t k = n - given < end - next ? n - given : end - next;
+      if (rank >= given && rank - given < k) item = next + (rank - given);
+      next += k;
+      given += k;
     }
     return item;
   }

@@ -166,9 +166,19 @@ constexpr uint32_t kPlainMaxWords = 2048;  // reps of at most 65,534 residues
 void launch_align_plain_tiers(const AlignArgs& a, uint32_t* jobs, uint32_t count,
                               uint32_t* scratch, uint64_t scratch_words, uint32_t nw_cap,
                               hipStream_t stream);
+// Work queue of a bin: items [0, count) in kQueueSlices contiguous slices,
+// one claim counter per slice, each on a line of its own
+// (queue[s * kQueueStride]); a queue is kQueueWords zeroed words.  A
+// wavefront claims from the slice of its XCD (blockIdx.x mod 8 under the
+// round-robin dispatch of workgroups to XCDs) and, once that is exhausted,
+// from the others: one counter for every wavefront of a bin serialised the
+// claims of bins with millions of short pairs.
+constexpr uint32_t kQueueSlices = 8;
+constexpr uint32_t kQueueStride = 64;  // words: 256 bytes
+constexpr uint32_t kQueueWords = kQueueSlices * kQueueStride;
 // Length-binned forms over a pair list (pairs[list[*off_ptr + i]], i <
-// *count_ptr); work is a zeroed work-queue counter; blocks is the persistent
-// grid.
+// *count_ptr); work is a zeroed work queue (kQueueWords words); blocks is the
+// persistent grid.
 size_t win_lds_bytes(int W, bool compact = false);
 size_t wave_lds_bytes(int K);
 void launch_align_win(int W, const AlignArgs& a, const uint2* pairs, const uint32_t* list,

@@ -477,19 +477,19 @@ void launch_batch(ngia_serve* sv, uint32_t k, const uint8_t* res, const uint64_t
     if ((1u << kk) > sv->ctx->wave_kmax) continue;
     const int bin = kBinWave0 + kk;
     launch_align_wave(1 << kk, aa, ln.pairs.as<uint2>(), ln.list.as<uint32_t>(), &dc->bin_off[bin],
-                      &dc->bin_cnt[bin], &dc->work[bin], ln.score.as<uint32_t>(),
+                      &dc->bin_cnt[bin], dc->queue[bin], ln.score.as<uint32_t>(),
                       std::min<uint32_t>(Q * 64, persistent_blocks(sv->ctx, wave_lds_bytes(1 << kk))), s);
   }
   for (int G = 32; G >= 16; G -= 16) {
     const int bin = G == 16 ? kBinSeg16 : kBinSeg32;
     launch_align_seg(G, aa, ln.pairs.as<uint2>(), ln.list.as<uint32_t>(), &dc->bin_off[bin],
-                     &dc->bin_cnt[bin], &dc->work[bin], ln.score.as<uint32_t>(),
+                     &dc->bin_cnt[bin], dc->queue[bin], ln.score.as<uint32_t>(),
                      std::min<uint32_t>(Q * 64, sv->ctx->prop.multiProcessorCount * seg_blocks_per_cu(G)),
                      s);
   }
   for (int W = kWinMax; W >= 2; W--)
     launch_align_win(W, aa, ln.pairs.as<uint2>(), ln.list.as<uint32_t>(), &dc->bin_off[W],
-                     &dc->bin_cnt[W], &dc->work[W], ln.score.as<uint32_t>(),
+                     &dc->bin_cnt[W], dc->queue[W], ln.score.as<uint32_t>(),
                      std::min<uint32_t>(Q, persistent_blocks(sv->ctx, win_lds_bytes(W))), s);
   plain_bin_jobs(ln.pairs.as<uint2>(), ln.list.as<uint32_t>(), dc->bin_off, dc->bin_cnt,
                  ln.plain_jobs.as<uint32_t>(), s);

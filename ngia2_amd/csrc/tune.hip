@@ -34,7 +34,7 @@ float time_kernel(ngia_ctx* ctx, const std::function<void()>& launch, uint32_t* 
   hipStream_t s = ctx->stream;
   float best = 1e30f;
   for (int rep = 0; rep < 3; rep++) {
-    fill_u32(work, 0, 1, s);
+    fill_u32(work, 0, kQueueWords, s);
     NGIA_HIP(hipEventRecord(ctx->ev[6], s));
     launch();
     NGIA_HIP(hipEventRecord(ctx->ev[7], s));
@@ -98,7 +98,7 @@ extern "C" int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* 
     pairs.ensure(8ull * kTunePairs);
     list.ensure(4ull * kTuneList);
     score.ensure(4ull * kTunePairs);
-    ctr.ensure(64);  // [0] offset, [1] count, [2] work
+    ctr.ensure(4ull * (kQueueStride + kQueueWords));  // [0] offset, [1] count; the work queue
     std::vector<uint2> hp(kTunePairs);
     std::vector<uint32_t> hl(kTuneList);
     for (uint32_t i = 0; i < kTunePairs; i++) hp[i] = make_uint2(2 * i, 2 * i + 1);
@@ -107,7 +107,7 @@ extern "C" int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* 
     h2d(list.p, hl.data(), 4ull * kTuneList, s);
     uint32_t* off = ctr.as<uint32_t>();
     uint32_t* cnt = off + 1;
-    uint32_t* work = off + 2;
+    uint32_t* work = off + kQueueStride;
     uint32_t np = 0;
     ngia::RouteTable route{}, route_compact{};
     for (int k = 0; k < kNumCounts; k++) route.class_size[k] = route_compact.class_size[k] = kTuneCounts[k];
