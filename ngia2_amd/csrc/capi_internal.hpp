@@ -334,6 +334,7 @@ inline AlignArgs align_args(const ngia_db* db, float thr) {
   a.word_off = db->word_off.as<uint64_t>();
   a.codes = db->codes.as<uint8_t>();
   a.code_off = db->code_off.as<uint64_t>();
+  a.lengths = db->lengths.as<uint32_t>();
   a.thr = thr;
   a.raw_lcs = db->ctx && db->ctx->raw_lcs ? 1u : 0u;
   if (db->compact) {

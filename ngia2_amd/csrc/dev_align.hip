@@ -573,7 +573,7 @@ __global__ __launch_bounds__(64) void k_align_win(AlignArgs A, const uint2* __re
           const uint2 pr = pairs[p];
           const uint32_t* rr = A.packed + A.word_off[pr.x];
           R = rr[0];
-          T = A.packed[A.word_off[pr.y]];
+          T = A.lengths[pr.y];
           const Band bd = band_edges(R, T, A.thr);
           const int32_t a = (int32_t)(bd.lsft / 32);
           const int32_t nw = (int32_t)((R + 31) / 32);
@@ -706,7 +706,7 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
     const uint2 pr = pairs[p];
     const uint32_t* rr = A.packed + A.word_off[pr.x];
     const uint32_t R = rr[0];
-    const uint32_t T = A.packed[A.word_off[pr.y]];
+    const uint32_t T = A.lengths[pr.y];
     const Band bd = band_edges(R, T, A.thr);
     const int32_t a = (int32_t)(bd.lsft / 32);
     const int32_t c = (int32_t)((bd.rsft + 31) / 32);
@@ -918,7 +918,7 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
           const uint2 pr = pairs[p];
           rr = A.packed + A.word_off[pr.x];
           R = rr[0];
-          T = A.packed[A.word_off[pr.y]];
+          T = A.lengths[pr.y];
           const Band bd = band_edges(R, T, A.thr);
           a = (int32_t)(bd.lsft / 32);
           c = (int32_t)((bd.rsft + 31) / 32);
@@ -1050,6 +1050,11 @@ size_t win_lds_bytes(int W, bool compact) {
 size_t wave_lds_bytes(int K) { return (size_t)16 * 64 * K * 4; }
 size_t seg_lds_bytes() { return (size_t)16 * 64 * 4; }
 
+// The binned kernels read target lengths from a.lengths.
+static void require_lengths(const AlignArgs& a) {
+  if (!a.lengths) throw HipError{NGIA_ERR_ARG, "binned alignment without record lengths"};
+}
+
 #define NGIA_WIN_CASE(Wv)                                                                  \
   case Wv:                                                                                 \
     if (a.slots)                                                                           \
@@ -1063,6 +1068,7 @@ size_t seg_lds_bytes() { return (size_t)16 * 64 * 4; }
 void launch_align_win(int W, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
                       const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work, uint32_t* score,
                       uint32_t blocks, hipStream_t stream) {
+  require_lengths(a);
   switch (W) {
     NGIA_WIN_CASE(2)
     NGIA_WIN_CASE(3)
@@ -1118,6 +1124,7 @@ uint32_t seg_blocks_per_cu(int G) {
 void launch_align_seg(int G, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
                       const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work,
                       uint32_t* score, uint32_t blocks, hipStream_t stream) {
+  require_lengths(a);
   if (G == 16)
     hipLaunchKernelGGL(k_align_seg<16>, dim3(blocks), dim3(64), 0, stream, a, pairs, list, off_ptr,
                        count_ptr, work, score);
@@ -1138,6 +1145,7 @@ void launch_align_seg(int G, const AlignArgs& a, const uint2* pairs, const uint3
 void launch_align_wave(int K, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
                        const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work, uint32_t* score,
                        uint32_t blocks, hipStream_t stream) {
+  require_lengths(a);
   switch (K) {
     NGIA_WAVE_CASE(1)
     NGIA_WAVE_CASE(2)

@@ -120,6 +120,10 @@ struct AlignArgs {
   const uint64_t* word_off;
   const uint8_t* codes;
   const uint64_t* code_off;
+  // Record lengths, as in each record's header word.  The binned kernels
+  // read a target's length here: its packed record is otherwise untouched,
+  // and its header would cost a line of memory traffic per pair.
+  const uint32_t* lengths = nullptr;
   float thr;
   uint32_t raw_lcs = 0;  // nonzero: store the LCS length instead of the score
   // Compact alphabet (a database whose residues use at most kSlotCodes

@@ -472,6 +472,7 @@ void launch_batch(ngia_serve* sv, uint32_t k, const uint8_t* res, const uint64_t
   aa.word_off = sv->word_off.as<uint64_t>();
   aa.codes = sv->codes.as<uint8_t>();
   aa.code_off = sv->code_off.as<uint64_t>();
+  aa.lengths = sv->lengths.as<uint32_t>();
   aa.thr = sv->thr;
   for (int kk = 5; kk >= 0; kk--) {
     if ((1u << kk) > sv->ctx->wave_kmax) continue;
