@@ -39,10 +39,10 @@ void finalize(ngia_ctx* ctx, ngia_db* db, uint64_t m) {
   finalize_edges(db->ekeys.as<uint64_t>(), db->escores.as<uint32_t>(), m,
                  db->ekeys_tmp.as<uint64_t>(), db->escores_tmp.as<uint32_t>(), db->fin_temp.p,
                  db->fin_temp.bytes, &dc->unique, ctx->stream);
-  unsigned long long u = 0;
-  d2h(&u, &dc->unique, 8, ctx->stream);
+  unsigned long long* u = (unsigned long long*)(ctx->pin + kPinUnique);
+  d2h(u, &dc->unique, 8, ctx->stream);
   NGIA_HIP(hipStreamSynchronize(ctx->stream));
-  db->n_edges = u;
+  db->n_edges = *u;
 }
 
 // Workspace of the plain form: per-band flags, scan, job triplets and the
@@ -358,8 +358,9 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
     NGIA_HIP(hipGraphLaunch(db->gexec, s));
   // the bins that hold pairs pick the second part
   uint32_t cnt[kBins];
-  d2h(cnt, db->counters.as<Counters>()->bin_cnt, sizeof(cnt), s);
+  d2h(ctx->pin + kPinCnt, db->counters.as<Counters>()->bin_cnt, sizeof(cnt), s);
   NGIA_HIP(hipStreamSynchronize(s));
+  std::memcpy(cnt, ctx->pin + kPinCnt, sizeof(cnt));
   uint32_t mask = 0;
   for (int b = 0; b < kBins; b++)
     if (cnt[b]) mask |= 1u << b;
@@ -393,15 +394,21 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   }
   NGIA_HIP(hipEventRecord(ctx->ev[1], s));
   Counters hc, hh{};
-  d2h(&hc, db->counters.p, kCounterStats, s);
   uint64_t npairs = 0, hpairs = 0;
-  d2h(&npairs, db->off.as<uint64_t>() + n, 8, s);
+  d2h(ctx->pin + kPinCounters, db->counters.p, kCounterStats, s);
+  d2h(ctx->pin + kPinPairs, db->off.as<uint64_t>() + n, 8, s);
   if (head) {
     NGIA_HIP(hipStreamWaitEvent(s, head->done, 0));
-    d2h(&hh, head->db->counters.p, kCounterStats, s);
-    d2h(&hpairs, head->db->off.as<uint64_t>() + head->db->n, 8, s);
+    d2h(ctx->pin + kPinHeadCounters, head->db->counters.p, kCounterStats, s);
+    d2h(ctx->pin + kPinHeadPairs, head->db->off.as<uint64_t>() + head->db->n, 8, s);
   }
   NGIA_HIP(hipStreamSynchronize(s));
+  std::memcpy(&hc, ctx->pin + kPinCounters, kCounterStats);
+  std::memcpy(&npairs, ctx->pin + kPinPairs, 8);
+  if (head) {
+    std::memcpy(&hh, ctx->pin + kPinHeadCounters, kCounterStats);
+    std::memcpy(&hpairs, ctx->pin + kPinHeadPairs, 8);
+  }
   db->n_pairs = npairs;
   NGIA_HIP(hipEventRecord(ctx->ev[4], s));
   if (hh.ecount) {
@@ -590,7 +597,7 @@ void graph_step(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands, b
     launch_sketch(codes, coff, len, n, sig, s);
   }
   launch_bucket(sig, n, db->rep.as<uint32_t>(), db->bucket_scratch.p, s, ctx->bucket_wide,
-                db->compact);
+                db->compact, (uint32_t*)(ctx->pin + kPinOverflow));
   db->have_sig = true;
   db->rep_bands = NGIA_BANDS;
   NGIA_HIP(hipEventRecord(ctx->ev[7], s));
@@ -638,6 +645,8 @@ extern "C" int ngia_open(int device, ngia_ctx** out) {
     const char* fa = std::getenv("NGIA_FULL_ALPHABET");
     ctx->full_alphabet = fa && fa[0] == '1';
     NGIA_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    NGIA_HIP(hipHostMalloc((void**)&ctx->pin, kPinBytes, hipHostMallocDefault));
+    std::memset(ctx->pin, 0, kPinBytes);
     for (auto& e : ctx->ev) NGIA_HIP(hipEventCreate(&e));
     for (auto& q : ctx->side) NGIA_HIP(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
     NGIA_HIP(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
@@ -749,7 +758,7 @@ extern "C" int ngia_db_bucket(ngia_ctx* ctx, ngia_db* db) {
     db->bucket_scratch.ensure(bucket_scratch_bytes(db->n, ctx->bucket_wide));
     NGIA_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
     launch_bucket(db->sig.as<uint32_t>(), db->n, db->rep.as<uint32_t>(), db->bucket_scratch.p,
-                  ctx->stream, ctx->bucket_wide, db->compact);
+                  ctx->stream, ctx->bucket_wide, db->compact, (uint32_t*)(ctx->pin + kPinOverflow));
     NGIA_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
     NGIA_HIP(hipStreamSynchronize(ctx->stream));
     ctx->timing.bucket_ms = event_ms(ctx->ev[6], ctx->ev[7]);

@@ -76,6 +76,11 @@ struct Counters {
   alignas(256) uint32_t queue[kBins][kQueueWords];
 };
 constexpr size_t kCounterStats = offsetof(Counters, queue);
+// Offsets into ngia_ctx::pin.
+constexpr size_t kPinOverflow = 0, kPinUnique = 8, kPinPairs = 16, kPinHeadPairs = 24,
+                 kPinCnt = 64, kPinCounters = 256, kPinHeadCounters = kPinCounters + kCounterStats,
+                 kPinBytes = kPinHeadCounters + kCounterStats;
+static_assert(kPinCnt + 4 * kBins <= kPinCounters, "pinned slots overlap");
 
 }  // namespace ngia
 
@@ -116,6 +121,13 @@ struct ngia_ctx {
   ngia_timing timing{};    // device time of the last sketch / bucket / graph
   // this device's copy of the row scratch of ngia_align_noalloc (noalloc.hip)
   uint32_t* noalloc_scratch = nullptr;
+  // Pinned host staging for the small device-to-host reads of a round (bin
+  // counts, counters, the edge count, the bucketing's overflow flag): a copy
+  // into pageable memory is staged and waited for one at a time, about 30 us
+  // each; into pinned memory the copies queue and one synchronisation covers
+  // them.  Slots (bytes): kPinOverflow, kPinUnique, kPinCnt, kPinPairs,
+  // kPinHeadPairs, kPinCounters, kPinHeadCounters.
+  unsigned char* pin = nullptr;
   // live handles made on this context; ngia_close detaches them, so a handle
   // destroyed after its context only frees its own memory
   std::vector<ngia_db*> dbs;
@@ -133,6 +145,7 @@ struct ngia_ctx {
     for (auto& q : side)
       if (q) (void)hipStreamDestroy(q);
     if (stream) (void)hipStreamDestroy(stream);
+    if (pin) (void)hipHostFree(pin);
   }
 };
 

@@ -86,8 +86,11 @@ __host__ __device__ inline uint32_t lcs_score(uint32_t lcs, uint32_t R, uint32_t
 // them, have at most 5^8 values and their minima crowd near the smallest
 // hashes) make long runs of equal signatures that overflow any partition.
 size_t bucket_scratch_bytes(uint32_t n, bool force_wide = false);
+// pinned_flag (optional): 4 bytes of pinned host memory that receive the
+// overflow flag of the partitioned form.
 void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch,
-                   hipStream_t stream, bool force_wide = false, bool few_values = false);
+                   hipStream_t stream, bool force_wide = false, bool few_values = false,
+                   uint32_t* pinned_flag = nullptr);
 // The same rows for n <= kBucketSmallMax records: one block per band, no
 // scratch and no host synchronisation.
 constexpr uint32_t kBucketSmallMax = 8192;

@@ -508,7 +508,7 @@ namespace {
 // The partitioned bucketing; returns false when a partition overflowed (rep
 // then holds a partial result and the caller redoes it with the full sort).
 bool bucket_partitioned(const PartLayout& L, const uint32_t* sig, uint32_t n, uint32_t* rep,
-                        char* s, hipStream_t stream) {
+                        char* s, hipStream_t stream, uint32_t* pinned_flag) {
   const uint64_t m = (uint64_t)n * 64;
   const uint32_t pb = L.bits, parts = 64u << pb;
   uint16_t* keys_in = (uint16_t*)(s + L.keys_in);
@@ -541,15 +541,16 @@ bool bucket_partitioned(const PartLayout& L, const uint32_t* sig, uint32_t n, ui
     hipLaunchKernelGGL((k_part_local<256, 4, 8, 16>), dim3(parts), dim3(256), 0, stream, vals_out,
                        start, end, n, pb, rep, overflow);
   NGIA_HIP(hipGetLastError());
-  uint32_t over = 0;
-  NGIA_HIP(hipMemcpyAsync(&over, overflow, 4, hipMemcpyDeviceToHost, stream));
+  uint32_t local = 0;
+  uint32_t* over = pinned_flag ? pinned_flag : &local;
+  NGIA_HIP(hipMemcpyAsync(over, overflow, 4, hipMemcpyDeviceToHost, stream));
   NGIA_HIP(hipStreamSynchronize(stream));
-  return over == 0;
+  return *over == 0;
 }
 }  // namespace
 
 void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch,
-                   hipStream_t stream, bool force_wide, bool few_values) {
+                   hipStream_t stream, bool force_wide, bool few_values, uint32_t* pinned_flag) {
   if (n == 0) return;
   const BucketLayout L = bucket_layout(n, force_wide);
   char* s = (char*)scratch;
@@ -561,7 +562,7 @@ void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch
   NGIA_HIP(hipGetLastError());
   const PartLayout P = part_layout(n, force_wide);
   if (P.use && !few_values) {
-    if (bucket_partitioned(P, sig, n, rep, s, stream)) return;
+    if (bucket_partitioned(P, sig, n, rep, s, stream, pinned_flag)) return;
     // a partition overflowed: start again from the identity rows, full sort
     hipLaunchKernelGGL(k_rep_identity, dim3(grid_for(n, 256), 64), dim3(256), 0, stream, n, rep);
     NGIA_HIP(hipGetLastError());
