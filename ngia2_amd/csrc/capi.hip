@@ -451,18 +451,30 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   }
 }
 
+// The database's representative rows from its signatures: one block per band
+// (no scratch, no host synchronisation) up to kBucketSmallMax records, the
+// partitioned or full sort beyond (or always, under NGIA_BUCKET_WIDE).
+void bucket_rows(ngia_ctx* ctx, ngia_db* db, hipStream_t s) {
+  if (!ctx->bucket_wide && db->n <= kBucketSmallMax)
+    launch_bucket_small(db->sig.as<uint32_t>(), db->n, db->rep.as<uint32_t>(), s);
+  else
+    launch_bucket(db->sig.as<uint32_t>(), db->n, db->rep.as<uint32_t>(), db->bucket_scratch.p, s,
+                  ctx->bucket_wide, db->compact, (uint32_t*)(ctx->pin + kPinOverflow));
+}
+
 // The head of a step: the leading run of records whose own band window (the
 // record against itself) is at least min_window words (64 by default, at
-// least 32) at thr, in non-increasing length order, at most kBucketSmallMax
-// of them.  A pair (r, i) of the head then has R >= T and a window at least
+// least 32) at thr, in non-increasing length order, at most kHeadMax of
+// them.  A pair (r, i) of the head then has R >= T and a window at least
 // that of (T, T): every head pair goes to a wavefront bin (or, past the
 // widest, the plain bin).  With 64 the head holds the pairs of K >= 2, the
 // long latency-bound ones; with 32 (the K = 1 pairs too) the heavy-tailed
 // step measured the same, and the log-normal set's 8,192-record head at
 // -i 50 lost 1%; with 128, 11.6 against 11.3 ms on the heavy-tailed set.
+constexpr uint32_t kHeadMax = 8192;
 uint32_t head_records(const ngia_db* db, float thr, uint32_t min_window) {
   uint32_t h = 0;
-  const uint32_t cap = std::min<uint32_t>(db->n, kBucketSmallMax);
+  const uint32_t cap = std::min<uint32_t>(db->n, kHeadMax);
   while (h < cap) {
     const uint32_t L = db->host_len[h];
     if (h && L > db->host_len[h - 1]) break;
@@ -596,8 +608,7 @@ void graph_step(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands, b
   } else {
     launch_sketch(codes, coff, len, n, sig, s);
   }
-  launch_bucket(sig, n, db->rep.as<uint32_t>(), db->bucket_scratch.p, s, ctx->bucket_wide,
-                db->compact, (uint32_t*)(ctx->pin + kPinOverflow));
+  bucket_rows(ctx, db, s);
   db->have_sig = true;
   db->rep_bands = NGIA_BANDS;
   NGIA_HIP(hipEventRecord(ctx->ev[7], s));
@@ -757,8 +768,7 @@ extern "C" int ngia_db_bucket(ngia_ctx* ctx, ngia_db* db) {
     db->rep.ensure(256ull * std::max<uint32_t>(db->n, 1));
     db->bucket_scratch.ensure(bucket_scratch_bytes(db->n, ctx->bucket_wide));
     NGIA_HIP(hipEventRecord(ctx->ev[6], ctx->stream));
-    launch_bucket(db->sig.as<uint32_t>(), db->n, db->rep.as<uint32_t>(), db->bucket_scratch.p,
-                  ctx->stream, ctx->bucket_wide, db->compact, (uint32_t*)(ctx->pin + kPinOverflow));
+    bucket_rows(ctx, db, ctx->stream);
     NGIA_HIP(hipEventRecord(ctx->ev[7], ctx->stream));
     NGIA_HIP(hipStreamSynchronize(ctx->stream));
     ctx->timing.bucket_ms = event_ms(ctx->ev[6], ctx->ev[7]);

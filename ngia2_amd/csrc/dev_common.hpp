@@ -93,7 +93,7 @@ void launch_bucket(const uint32_t* sig, uint32_t n, uint32_t* rep, void* scratch
                    uint32_t* pinned_flag = nullptr);
 // The same rows for n <= kBucketSmallMax records: one block per band, no
 // scratch and no host synchronisation.
-constexpr uint32_t kBucketSmallMax = 8192;
+constexpr uint32_t kBucketSmallMax = 16384;
 void launch_bucket_small(const uint32_t* sig, uint32_t n, uint32_t* rep, hipStream_t stream);
 // Signatures of long records in chunks of `len` window positions:
 // chunks[c] = (record, first position); sig rows set to all ones beforehand.

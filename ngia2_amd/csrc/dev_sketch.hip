@@ -387,17 +387,19 @@ __global__ __launch_bounds__(BS) void k_part_local(const uint64_t* __restrict__ 
     atomicAdd(overflow, 1u);
 }
 
-// Bucketing of at most kBucketSmallMax records, for the head of a step: one
-// block per band sorts the band's n (mix(sig), i) pairs in LDS, stably, and
-// points each element equal to its predecessor at that predecessor, as
-// k_part_local does for a partition.  No global sort, no look-back between
-// blocks and no host synchronisation, so it runs beside other kernels.
+// Bucketing of at most kBucketSmallMax records (the head of a step, and any
+// database that small): one block per band sorts the band's n (mix(sig), i)
+// pairs in LDS, stably, and points each element equal to its predecessor at
+// that predecessor, as k_part_local does for a partition.  No global sort, no
+// look-back between blocks and no host synchronisation, so it runs beside
+// other kernels.  The sort's storage holds the threads' last items after the
+// sort (8 bytes per thread, well inside it).
 template <int BS, int ITEMS>
 __global__ __launch_bounds__(BS) void k_bucket_small(const uint32_t* __restrict__ sig, uint32_t n,
                                                      uint32_t* __restrict__ rep) {
   using Sort = rocprim::block_radix_sort<uint32_t, BS, ITEMS, uint32_t>;
+  static_assert(sizeof(typename Sort::storage_type) >= 8 * BS, "last items do not fit");
   __shared__ typename Sort::storage_type storage;
-  __shared__ uint32_t lastk[BS], lastv[BS];
   const uint32_t b = blockIdx.x, t = threadIdx.x;
   const uint64_t row = (uint64_t)b * n;
   uint32_t key[ITEMS], val[ITEMS];
@@ -415,6 +417,9 @@ __global__ __launch_bounds__(BS) void k_bucket_small(const uint32_t* __restrict_
   }
   __syncthreads();
   Sort().sort(key, val, storage, 0, 32);
+  __syncthreads();  // the storage is free again
+  uint32_t* lastk = reinterpret_cast<uint32_t*>(&storage);
+  uint32_t* lastv = lastk + BS;
   lastk[t] = key[ITEMS - 1];
   lastv[t] = val[ITEMS - 1];
   __syncthreads();
@@ -433,8 +438,10 @@ void launch_bucket_small(const uint32_t* sig, uint32_t n, uint32_t* rep, hipStre
   if (n > kBucketSmallMax) throw HipError{NGIA_ERR_ARG, "too many records for the small bucketing"};
   if (n <= 2048)
     hipLaunchKernelGGL((k_bucket_small<256, 8>), dim3(64), dim3(256), 0, stream, sig, n, rep);
-  else
+  else if (n <= 8192)
     hipLaunchKernelGGL((k_bucket_small<1024, 8>), dim3(64), dim3(1024), 0, stream, sig, n, rep);
+  else
+    hipLaunchKernelGGL((k_bucket_small<1024, 16>), dim3(64), dim3(1024), 0, stream, sig, n, rep);
   NGIA_HIP(hipGetLastError());
 }
 

@@ -94,9 +94,10 @@ def test_bucket_wide_keys(c1db):
 
 
 def test_bucket_small_blocks():
-    """The head's bucketing (one block per band, the sort in LDS), through
-    NGIA_BUCKET_SMALL: both block sizes and the capacity, 1 to 8,192 records,
-    with heavy collisions and all-ones signatures (the sort's padding key)."""
+    """The small bucketing (one block per band, the sort in LDS: a step's
+    head and databases of at most 16,384 records), through NGIA_BUCKET_SMALL:
+    the three block shapes and their capacities, 1 to 16,384 records, with
+    heavy collisions and all-ones signatures (the sort's padding key)."""
     rng = np.random.default_rng(17)
     os.environ["NGIA_BUCKET_SMALL"] = "1"
     try:
@@ -104,7 +105,7 @@ def test_bucket_small_blocks():
     finally:
         del os.environ["NGIA_BUCKET_SMALL"]
     with c2:
-        for n in (1, 2, 100, 2048, 2049, 5000, 8192):
+        for n in (1, 2, 100, 2048, 2049, 5000, 8192, 8193, 12000, 16384):
             say("bucket small", n)
             sig = rng.integers(0, 1 << 32, size=(n, 64), dtype=np.uint64).astype(np.uint32)
             sig[:, 1] = rng.integers(0, 7, size=n)
