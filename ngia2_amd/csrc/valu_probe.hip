@@ -28,11 +28,14 @@ namespace {
   } while (0)
 
 enum Op {
-  kXor, kAdd, kShr, kMin, kMin3, kLshlAdd, kBitop3, kPerm, kAddCo, kAddcCo, kCmp, kMulLo, kOps
+  kXor, kAdd, kShr, kMin, kMin3, kLshlAdd, kBitop3, kPerm, kAddCo, kAddcCo, kCmp, kMulLo,
+  kCndmask, kMovSdwa, kAddCoVcc, kCmpVcc, kOps
 };
 const char* kNames[kOps] = {"v_xor_b32",      "v_add_u32",     "v_lshrrev_b32", "v_min_u32",
                             "v_min3_u32",     "v_lshl_add_u32", "v_bitop3_b32",  "v_perm_b32",
-                            "v_add_co_u32",   "v_addc_co_u32", "v_cmp_eq_u32",  "v_mul_lo_u32"};
+                            "v_add_co_u32",   "v_addc_co_u32", "v_cmp_eq_u32",  "v_mul_lo_u32",
+                            "v_cndmask_b32 (e64, SGPR mask)", "v_mov_b32_sdwa (byte 1 of dst)",
+                            "v_add_co_u32_e32 (carry to VCC)", "v_cmp_eq_u32_e32 (to VCC)"};
 
 template <int OP>
 __device__ __forceinline__ void step(uint32_t& a, uint64_t& m, uint32_t b, uint32_t c) {
@@ -51,6 +54,13 @@ __device__ __forceinline__ void step(uint32_t& a, uint64_t& m, uint32_t b, uint3
     asm volatile("v_addc_co_u32 %0, %1, %0, %2, %1" : "+v"(a), "+s"(m) : "v"(b));
   if constexpr (OP == kCmp) asm volatile("v_cmp_eq_u32 %0, %1, %2" : "=s"(m) : "v"(a), "v"(b));
   if constexpr (OP == kMulLo) asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(a) : "v"(b));
+  if constexpr (OP == kCndmask)
+    asm volatile("v_cndmask_b32_e64 %0, %0, %1, %2" : "+v"(a) : "v"(b), "s"(m));
+  if constexpr (OP == kAddCoVcc) asm volatile("v_add_co_u32_e32 %0, vcc, %0, %1" : "+v"(a) : "v"(b) : "vcc");
+  if constexpr (OP == kCmpVcc) asm volatile("v_cmp_eq_u32_e32 vcc, %0, %1" : : "v"(a), "v"(b) : "vcc");
+  if constexpr (OP == kMovSdwa)
+    asm volatile("v_mov_b32_sdwa %0, %1 dst_sel:BYTE_1 dst_unused:UNUSED_PRESERVE src0_sel:BYTE_2"
+                 : "+v"(a) : "v"(b));
 }
 
 template <int OP>
@@ -127,6 +137,10 @@ int main(int argc, char** argv) {
   run<kAddcCo>(out, blocks, iters, prop.clockRate, simds);
   run<kCmp>(out, blocks, iters, prop.clockRate, simds);
   run<kMulLo>(out, blocks, iters, prop.clockRate, simds);
+  run<kCndmask>(out, blocks, iters, prop.clockRate, simds);
+  run<kMovSdwa>(out, blocks, iters, prop.clockRate, simds);
+  run<kAddCoVcc>(out, blocks, iters, prop.clockRate, simds);
+  run<kCmpVcc>(out, blocks, iters, prop.clockRate, simds);
   CHECK(hipFree(out));
   return 0;
 }
