@@ -205,6 +205,35 @@ def test_step_split_equals_stages(split_ctx, config2, config4, config1, cfg, ide
     db.close()
 
 
+def test_step_rows_after_partition_overflow(ctx):
+    """A step whose partitioned bucketing overflows: 9,000 copies of one
+    record among 20,000 put 9,000 equal signatures into one partition of
+    every band, past its 8,192 capacity, and the step's bucketing redoes the
+    rows with the full sort: signatures, rows, candidate count and edges
+    equal the oracle's, in two steps in a row."""
+    rng = np.random.default_rng(23)
+    aa = np.frombuffer(b"ACDEFGHIKLMNPQRSTVWY", dtype=np.uint8)
+    common = aa[rng.integers(0, 20, 120)].tobytes()
+    seqs = [common] * 9000 + [aa[rng.integers(0, 20, int(rng.integers(80, 200)))].tobytes()
+                              for _ in range(11000)]
+    seqs = [seqs[i] for i in rng.permutation(len(seqs))]
+    packed, word_off = orc.pack_many(seqs)
+    lengths = np.array([len(x) for x in seqs], dtype=np.uint32)
+    sig = orc.sketch_all(packed, word_off)
+    rep = orc.bucket(sig)
+    want, tot = orc.build_graph(packed, word_off, lengths, rep, 64, 90)
+    assert len(want) >= 8999
+    say("step with a partition overflow, records", len(seqs))
+    db = ng.Database(ctx, packed, word_off, lengths)
+    for _ in range(2):
+        st = db.step(90, 64)
+        np.testing.assert_array_equal(db.sigs(), sig)
+        np.testing.assert_array_equal(db.reps(), rep)
+        np.testing.assert_array_equal(db.edges(), want)
+        assert st["total_pairs"] == tot
+    db.close()
+
+
 def test_config5_serving(ctx):
     """Serving over a config-4 resident set (2% of the families, with its
     Pareto tail) clustered on the device (batched graph + Leiden): 3,000
