@@ -36,7 +36,7 @@ void ensure_edge_space(ngia_db* db, uint64_t m) {
 
 void finalize(ngia_ctx* ctx, ngia_db* db, uint64_t m) {
   Counters* dc = db->counters.as<Counters>();
-  finalize_edges(db->ekeys.as<uint64_t>(), db->escores.as<uint32_t>(), m,
+  finalize_edges(db->ekeys.as<uint64_t>(), db->escores.as<uint32_t>(), m, db->n,
                  db->ekeys_tmp.as<uint64_t>(), db->escores_tmp.as<uint32_t>(), db->fin_temp.p,
                  db->fin_temp.bytes, &dc->unique, ctx->stream);
   unsigned long long* u = (unsigned long long*)(ctx->pin + kPinUnique);

@@ -552,23 +552,25 @@ void pairs_to_edges(const uint2* pairs, const uint64_t* npairs, const uint32_t* 
 size_t finalize_scratch_bytes(uint64_t m) {
   size_t a = 0, b = 0;
   (void)rocprim::radix_sort_pairs(nullptr, a, (uint64_t*)nullptr, (uint64_t*)nullptr,
-                                  (uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)m, 0, 63);
+                                  (uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)m, 0, 64);
   (void)rocprim::unique_by_key(nullptr, b, (uint64_t*)nullptr, (uint32_t*)nullptr,
                                (uint64_t*)nullptr, (uint32_t*)nullptr,
                                (unsigned long long*)nullptr, (size_t)m);
   return (a > b ? a : b) + 256;
 }
 
-void finalize_edges(uint64_t* keys, uint32_t* scores, uint64_t m, uint64_t* keys_tmp,
+void finalize_edges(uint64_t* keys, uint32_t* scores, uint64_t m, uint32_t n, uint64_t* keys_tmp,
                     uint32_t* scores_tmp, void* temp, size_t temp_bytes,
                     unsigned long long* d_unique, hipStream_t s) {
   if (m == 0) {
     fill_u32((uint32_t*)d_unique, 0, 2, s);
     return;
   }
+  // bits of the largest record index: the keys' bits from 32 + ib on are 0
+  const uint32_t ib = n > 1 ? 32u - (uint32_t)__builtin_clz(n - 1) : 1u;
   size_t tb = temp_bytes;
-  NGIA_HIP(rocprim::radix_sort_pairs(temp, tb, keys, keys_tmp, scores, scores_tmp, (size_t)m, 0, 63,
-                                     s));
+  NGIA_HIP(rocprim::radix_sort_pairs(temp, tb, keys, keys_tmp, scores, scores_tmp, (size_t)m, 0,
+                                     32 + ib, s));
   tb = temp_bytes;
   NGIA_HIP(rocprim::unique_by_key(temp, tb, keys_tmp, scores_tmp, keys, scores, d_unique, (size_t)m,
                                   rocprim::equal_to<uint64_t>(), s));

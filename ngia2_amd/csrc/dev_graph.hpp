@@ -81,8 +81,10 @@ void pairs_to_edges(const uint2* pairs, const uint64_t* npairs, const uint32_t* 
                     hipStream_t s);
 
 // Sort + unique of m raw edges in place (keys/scores); *d_unique = count.
+// The keys are u << 32 | v with u, v < n: the sort skips the bits above
+// 32 + bits(n - 1).
 size_t finalize_scratch_bytes(uint64_t m);
-void finalize_edges(uint64_t* keys, uint32_t* scores, uint64_t m, uint64_t* keys_tmp,
+void finalize_edges(uint64_t* keys, uint32_t* scores, uint64_t m, uint32_t n, uint64_t* keys_tmp,
                     uint32_t* scores_tmp, void* temp, size_t temp_bytes,
                     unsigned long long* d_unique, hipStream_t s);
 
