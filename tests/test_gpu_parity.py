@@ -114,6 +114,41 @@ def test_bucket_small_blocks():
             np.testing.assert_array_equal(c2.bucket(sig), orc.bucket(sig))
 
 
+@pytest.mark.parametrize("n", [16384, 16385])
+def test_bucket_small_database_boundary(ctx, n):
+    """A database at the one-block-per-band bucketing's limit (16,384
+    records) and one past it (the partitioned form), in families of eight
+    close copies: signatures and rows of the stage calls, and the rows,
+    candidate count and edges of a step, equal the oracle's."""
+    rng = np.random.default_rng(n)
+    aa = np.frombuffer(AMINO, dtype=np.uint8)
+    seqs = []
+    while len(seqs) < n:
+        anc = aa[rng.integers(0, 20, int(rng.integers(60, 160)))]
+        for _ in range(min(8, n - len(seqs))):
+            x = anc.copy()
+            sub = rng.random(len(x)) < 0.05
+            x[sub] = aa[rng.integers(0, 20, int(sub.sum()))]
+            seqs.append(x.tobytes())
+    packed, word_off = orc.pack_many(seqs)
+    lengths = np.array([len(x) for x in seqs], dtype=np.uint32)
+    sig = orc.sketch_all(packed, word_off)
+    rep = orc.bucket(sig)
+    want, tot = orc.build_graph(packed, word_off, lengths, rep, 64, 90)
+    assert len(want) > n // 8
+    say("bucket and step at the small-database boundary", n)
+    db = ng.Database(ctx, packed, word_off, lengths)
+    db.sketch()
+    db.bucket()
+    np.testing.assert_array_equal(db.sigs(), sig)
+    np.testing.assert_array_equal(db.reps(), rep)
+    st = db.step(90, 64)
+    np.testing.assert_array_equal(db.reps(), rep)
+    np.testing.assert_array_equal(db.edges(), want)
+    assert st["total_pairs"] == tot
+    db.close()
+
+
 def test_bucket_collisions(ctx):
     # heavy signature collisions: few distinct values per band
     rng = np.random.default_rng(5)
