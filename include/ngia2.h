@@ -240,6 +240,8 @@ int ngia_db_band_jobs(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ba
 #define NGIA_BIN_WAVE 2u     /* k_align_wave<K>: one wavefront per pair; width = K */
 #define NGIA_BIN_PLAIN 3u
 #define NGIA_BIN_SEG 4u      /* k_align_seg<G>: 64/G pairs per wavefront; width = G */
+#define NGIA_BIN_SEG2 5u     /* k_align_seg<G, 2>: 64/G pairs per wavefront, two words per
+                              * lane; width = G */
 typedef struct {
   uint32_t kind;       /* NGIA_BIN_* */
   uint32_t width;      /* W (window), K (wave), G (seg); 0 otherwise */

@@ -164,7 +164,8 @@ void enqueue_bins(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands,
   const uint32_t grid = persistent_blocks(ctx, 0) * 4;
   bin_pairs(db->pairs.as<uint2>(), npairs, len, thr, db->bin_of.as<uint32_t>(), dc->bin_cnt,
             dc->bin_off, dc->cursor, db->list.as<uint32_t>(), dc->bin_tiles, db->score.as<uint32_t>(),
-            ctx->wave_kmax, dc->remap, db->compact ? ctx->route_compact : ctx->route, grid, s);
+            ctx->wave_kmax, dc->remap, db->compact ? ctx->route_compact : ctx->route, ctx->seg2_min,
+            grid, s);
 }
 
 // Part 2: the alignment kernels of the bins in `mask` (bit b: bin b holds
@@ -221,11 +222,13 @@ void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
   auto seg = [&](int bin, hipStream_t q) {
     if (!holds(bin)) return;
     mark(bin, 0, q);
-    launch_align_seg(bin == kBinSeg16 ? 16 : 32, args(bin), db->pairs.as<uint2>(), db->list.as<uint32_t>(),
+    const int G = bin == kBinSeg32 ? 32 : 16, K = bin == kBinSeg16x2 ? 2 : 1;
+    launch_align_seg(G, K, args(bin), db->pairs.as<uint2>(), db->list.as<uint32_t>(),
                      &dc->bin_off[bin], &dc->bin_cnt[bin], dc->queue[bin], db->score.as<uint32_t>(),
-                     ctx->prop.multiProcessorCount * seg_blocks_per_cu(bin == kBinSeg16 ? 16 : 32), q);
+                     ctx->prop.multiProcessorCount * seg_blocks_per_cu(G, K), q);
     mark(bin, 1, q);
   };
+  seg(kBinSeg16x2, fork ? ctx->side[0] : s);
   seg(kBinSeg32, fork ? ctx->side[0] : s);
   seg(kBinSeg16, fork ? ctx->side[3] : s);
   static constexpr int kWinStreams[3] = {0, 3, 2};
@@ -342,6 +345,7 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
       for (int k = 0; k < kTuneClasses; k++) key.push_back(rt.shape[W][k]);
   }
   key.push_back(db->t0);
+  key.push_back(ctx->seg2_min);
   const bool reuse = db->gexec && db->g_identity == identity && db->g_bands == bands &&
                      db->g_key == key;
   if (!eager && !reuse) {
@@ -429,7 +433,7 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   db->have_bins = true;
   uint32_t launches = 0;
   for (int b = 0; b < kBins; b++) {
-    const bool aligner = b == kBinPlain || b == kBinSeg16 || b == kBinSeg32 ||
+    const bool aligner = b == kBinPlain || b == kBinSeg16 || b == kBinSeg32 || b == kBinSeg16x2 ||
                          (b >= 2 && b <= kWinMax) ||
                          (b >= kBinWave0 && b < kBinPlain && (1u << (b - kBinWave0)) <= ctx->wave_kmax);
     const bool launched = aligner && ((mask >> b) & 1u);
@@ -649,6 +653,8 @@ extern "C" int ngia_open(int device, ngia_ctx** out) {
     if (ctx->head_min < 1) ctx->head_min = 1;
     const char* bs = std::getenv("NGIA_BUCKET_SMALL");
     ctx->bucket_small = bs && bs[0] == '1';
+    const char* s2 = std::getenv("NGIA_SEG2_MIN");
+    ctx->seg2_min = s2 ? (uint32_t)std::strtoul(s2, nullptr, 10) : kSeg2MinPairs;
     const char* sb = std::getenv("NGIA_SERIAL_BINS");
     ctx->serial_bins = sb && sb[0] == '1';
     const char* rl = std::getenv("NGIA_RAW_LCS");
@@ -943,6 +949,9 @@ extern "C" int ngia_db_bin_stats(ngia_ctx* ctx, ngia_db* db, ngia_bin_stat* out,
     } else if (b == kBinPlain) {
       o.kind = NGIA_BIN_PLAIN;
       o.width = 0;
+    } else if (b == kBinSeg16x2) {
+      o.kind = NGIA_BIN_SEG2;
+      o.width = 16u;
     } else {
       o.kind = NGIA_BIN_SEG;
       o.width = b == kBinSeg16 ? 16u : 32u;

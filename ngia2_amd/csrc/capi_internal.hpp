@@ -114,6 +114,10 @@ struct ngia_ctx {
   // NGIA_SERIAL_BINS=1 runs the alignment bins one after another on the
   // context stream (each bin's kernel time is then its time alone)
   bool serial_bins = false;
+  // Bins of 17-30-word windows with fewer pairs than this go to the 32-lane
+  // kernel, one word per lane (env NGIA_SEG2_MIN; 0 keeps every such bin on
+  // two words per lane)
+  uint32_t seg2_min = ngia::kSeg2MinPairs;
   bool raw_lcs = false;  // env NGIA_RAW_LCS=1: alignment kernels store LCS lengths, not scores
   // env NGIA_FULL_ALPHABET=1: the window kernels keep the 16-code table even
   // for a compact alphabet

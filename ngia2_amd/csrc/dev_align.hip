@@ -857,23 +857,31 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
 }
 
 // ------------------------------------------------ several pairs per wavefront
-// Band windows of at most G - 1 words (G = 16 or 32): one pair per G-lane
-// segment (64 / G pairs per wavefront), one word per lane.  In the segment of
-// lanes [base, base + G), lanes base .. base + G - 2 hold the words
-// [B, B + G - 1) of its pair's rep, B = cb - a, the band bottom; the band
-// [cb - a, cb + c) moves up one word per block, so a segment's registers
-// shift down one lane after every block (K = 1 of the wavefront kernel).
-// Lane base + G - 1 is a sentinel whose row restarts at 0 each block: within
-// a block of 32 characters that row stays below 2^31 (a row at most doubles
-// plus one per character), so it neither generates nor propagates a carry.
-// One 64-lane carry lookahead per character thus serves every segment, and
-// no carry crosses from one segment into the next.  The sentinel's slot is
-// the one the word that left last held (cleared), so its mask is 0.
+// One pair per G-lane segment (64 / G pairs per wavefront), K words per lane
+// (K = 1 or 2), for band windows of at most K (G - 1) words.  In the segment
+// of lanes [base, base + G), lane base + l holds the words B + K l + m,
+// m < K, of its pair's rep; B = K floor((cb - a) / K) trails the band bottom
+// cb - a, so the segment's registers shift down one lane every K blocks
+// (as in the wavefront kernel), and the band [cb - a, cb + c) lies inside the
+// held words [B, B + K G - 1).  The top lane's last word is a sentinel whose
+// row restarts at 0 each block: within a block of 32 characters that row
+// stays below 2^31 (a row at most doubles plus one per character), so it
+// neither generates nor propagates a carry.  One 64-lane carry lookahead per
+// character thus serves every segment, and no carry crosses from one segment
+// into the next.
+// Match masks: K tables of [16 codes][64 slots]; word w of a segment is in
+// table w mod K at slot base + (floor(w / K) mod G), so a lane's K words sit
+// at one slot of the K tables, one v_perm address per character.  The
+// sentinel's slot is the one the word that left last held (cleared), so its
+// mask is 0.
+// With K = 2 a segment of 16 lanes takes windows of up to 30 words, which
+// K = 1 needs 32 lanes for: four pairs per wavefront share each character's
+// lookahead instead of two, and a lane's two words share its mask address.
 // Each segment takes the next pair from the wave's claim on the work queue
 // (WaveClaim) when its pair ends;
 // the segments run their blocks in lockstep, one without work holding code 0
 // (no change), and the wave leaves when every segment found the queue empty.
-template <int G>
+template <int G, int K>
 __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __restrict__ pairs,
                                                   const uint32_t* __restrict__ list,
                                                   const uint32_t* __restrict__ off_ptr,
@@ -881,25 +889,32 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
                                                   uint32_t* __restrict__ work,
                                                   uint32_t* __restrict__ score) {
   static_assert(G == 16 || G == 32, "segments of 16 or 32 lanes");
-  // [16 codes][64 slots]: the word w of the segment at lanes [base, base + G)
-  // has slot base + (w mod G); code 0 stays 0
-  __shared__ __align__(16) uint32_t peq[16 * 64];
+  static_assert(K == 1 || K == 2, "one or two words per lane");
+  constexpr int LG = K == 1 ? 0 : 1;
+  // [K][16 codes][64 slots]; code 0 stays 0
+  __shared__ __align__(16) uint32_t peq[K * 16 * 64];
   const uint32_t lane = threadIdx.x;
   const uint32_t l = lane & (uint32_t)(G - 1), base = lane - l;
   const uint32_t count = *count_ptr;
   const uint32_t* const lst = list + (off_ptr ? *off_ptr : 0u);
   __builtin_amdgcn_s_setprio(1);
-  for (uint32_t x = lane; x < 16 * 64; x += 64) peq[x] = 0u;
+  for (uint32_t x = lane; x < K * 16 * 64; x += 64) peq[x] = 0u;
   lds_order();
   // The state of this lane's segment (equal in its G lanes; lcs in lane base).
   bool active = false, need = true;
   uint32_t p = 0, R = 0, T = 0, nb = 0, cb = 0, lcs = 0;
-  int32_t a = 0, c = 0, nw = 0;
+  int32_t a = 0, c = 0, nw = 0, B = 0;
   const uint32_t* rr = A.packed;
   const uint4* tcq = (const uint4*)A.codes;  // the target's codes, 32 bytes per block
-  uint32_t row = 0, vs = base, q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+  uint32_t row[K], vs = base, q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+  #pragma unroll
+  for (int m = 0; m < K; m++) row[m] = 0u;
   uint4 c0 = make_uint4(0u, 0u, 0u, 0u), c1 = c0;  // this block's 32 codes, one byte each
   uint64_t ran = 0;  // tally (lane base): reference tiles of the blocks run
+  // the sentinel: the top lane's last word
+  auto sentinel = [&](int m) { return l == (uint32_t)(G - 1) && m == K - 1; };
+  // the slot of word w in table w mod K
+  auto slot_of = [&](int32_t w) { return base + ((uint32_t)(w >> LG) & (uint32_t)(G - 1)); };
   WaveClaim q(count, 64u / G, 4u * (64u / G));
   for (;;) {
     // Segments whose pair ended take the next ones; one that finds the queue
@@ -930,21 +945,31 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
             need = true;
           } else {
             // The segment's slots: cleared, then the words in the band at
-            // block 0, [0, min(c, nw)) with c <= G - 1, word w at slot base + w.
+            // block 0, [0, min(c, nw)) with c <= K (G - 1): word w = K l + m
+            // at slot base + l of table m.
             #pragma unroll
-            for (int k = 1; k < 16; k++) peq[k * 64 + lane] = 0u;
-            lds_order();
-            const int32_t top0 = c < nw ? c : nw;
-            if ((int32_t)l < top0) {
-              const uint32_t* pl = rr + 1 + 4 * l;
-              uint32_t m[16];
-              peq_masks(pl[0], pl[1], pl[2], pl[3], m);
+            for (int m = 0; m < K; m++) {
               #pragma unroll
-              for (int k = 1; k < 16; k++) peq[k * 64 + lane] = m[k];
+              for (int k = 1; k < 16; k++) peq[(m * 16 + k) * 64 + lane] = 0u;
             }
             lds_order();
-            row = l < (uint32_t)(G - 1) ? 0xFFFFFFFFu : 0u;
-            vs = base + ((uint32_t)((int32_t)l - a) & (uint32_t)(G - 1));  // slot of word B + l, B = -a
+            const int32_t top0 = c < nw ? c : nw;
+            #pragma unroll
+            for (int m = 0; m < K; m++) {
+              const int32_t w = (int32_t)(K * l) + m;
+              if (w < top0) {
+                const uint32_t* pl = rr + 1 + 4 * w;
+                uint32_t mk[16];
+                peq_masks(pl[0], pl[1], pl[2], pl[3], mk);
+                #pragma unroll
+                for (int k = 1; k < 16; k++) peq[(m * 16 + k) * 64 + lane] = mk[k];
+              }
+            }
+            lds_order();
+            B = ((-a) >> LG) * K;  // K * floor(-a / K)
+            #pragma unroll
+            for (int m = 0; m < K; m++) row[m] = sentinel(m) ? 0u : 0xFFFFFFFFu;
+            vs = base + ((uint32_t)((B >> LG) + (int32_t)l) & (uint32_t)(G - 1));
             lcs = 0;
             cb = 0;
             c0 = tcq[0];
@@ -973,45 +998,58 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
     const uint32_t vs4 = vs * 4;
     // The masks of group g's four characters; those of group g + 1 are
     // loaded while group g is computed (two buffers, g mod 2).
-    uint32_t Mb[2][4];
-    auto load_group = [&](int g, uint32_t (&Mx)[4]) {
+    uint32_t Mb[2][4][K];
+    auto load_group = [&](int g, uint32_t (&Mx)[4][K]) {
       #pragma unroll
       for (int mm = 0; mm < 4; mm++) {
-        // byte address code * 256 + slot * 4 (slot < 64)
+        // byte address code * 256 + slot * 4 (slot < 64); table m 4 KiB on
         const uint32_t addr =
             __builtin_amdgcn_perm(cw[g], vs4, 0x0C0C0000u | ((4u + (uint32_t)mm) << 8));
-        Mx[mm] = *(const uint32_t*)((const char*)peq + addr);
+        #pragma unroll
+        for (int m = 0; m < K; m++)
+          Mx[mm][m] = *(const uint32_t*)((const char*)peq + m * 4096 + addr);
       }
     };
     load_group(0, Mb[0]);
     #pragma unroll
     for (int g = 0; g < 8; g++) {
-      uint32_t(&Mq)[4] = Mb[g & 1];
+      uint32_t(&Mq)[4][K] = Mb[g & 1];
       if (g < 7) load_group(g + 1, Mb[(g + 1) & 1]);
       #pragma unroll
       for (int mm = 0; mm < 4; mm++) {
-        const uint32_t M = Mq[mm];
+        const uint32_t(&M)[K] = Mq[mm];
         // As in k_align_wave: the carry chain through a lane mask (the carry
         // operand of v_add_co / v_addc_co) and the ballot lookahead.
-        uint32_t s0;
+        uint32_t s0[K];
         uint64_t cy;
-        asm("v_add_co_u32 %0, %1, %2, %3" : "=v"(s0), "=s"(cy) : "v"(row), "v"(row & M));
+        asm("v_add_co_u32 %0, %1, %2, %3" : "=v"(s0[0]), "=s"(cy) : "v"(row[0]), "v"(row[0] & M[0]));
+        uint32_t all = s0[0];
+        #pragma unroll
+        for (int m = 1; m < K; m++) {
+          asm("v_addc_co_u32 %0, %1, %2, %3, %4"
+              : "=v"(s0[m]), "=s"(cy) : "v"(row[m]), "v"(row[m] & M[m]), "s"(cy));
+          all &= s0[m];
+        }
         const uint64_t G1 = cy;
-        const uint64_t P = __builtin_amdgcn_ballot_w64(s0 == 0xFFFFFFFFu);
+        const uint64_t P = __builtin_amdgcn_ballot_w64(all == 0xFFFFFFFFu);
         uint64_t ci = ((G1 | P) + G1) ^ P;
-        uint32_t sum;
-        asm("v_addc_co_u32 %0, %1, %2, 0, %3" : "=v"(sum), "=s"(ci) : "v"(s0), "s"(ci));
-        row = sum | (row & ~M);
-        asm("" : "+v"(row));
+        #pragma unroll
+        for (int m = 0; m < K; m++) {
+          uint32_t sum;
+          asm("v_addc_co_u32 %0, %1, %2, 0, %3" : "=v"(sum), "=s"(ci) : "v"(s0[m]), "s"(ci));
+          row[m] = sum | (row[m] & ~M[m]);
+          asm("" : "+v"(row[m]));
+        }
       }
     }
     // Word cb - a leaves the band, word cb + c enters it for block cb + 1.
     if (active) {
       const int32_t wl = (int32_t)cb - a, we = (int32_t)cb + c;
       if (l >= 1 && l < 16) {
-        if (wl >= 0 && wl < nw) peq[l * 64 + base + ((uint32_t)wl & (uint32_t)(G - 1))] = 0u;
+        if (wl >= 0 && wl < nw)
+          peq[((wl & (K - 1)) * 16 + (int32_t)l) * 64 + slot_of(wl)] = 0u;
         if (we < nw)
-          peq[l * 64 + base + ((uint32_t)we & (uint32_t)(G - 1))] = code_mask(q0, q1, q2, q3, l);
+          peq[((we & (K - 1)) * 16 + (int32_t)l) * 64 + slot_of(we)] = code_mask(q0, q1, q2, q3, l);
       }
       lds_order();
       if (we + 1 < nw) {
@@ -1019,19 +1057,37 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
         q0 = pl[0]; q1 = pl[1]; q2 = pl[2]; q3 = pl[3];
       }
     }
-    // B moves up one word: the word in lane base leaves (counted there), the
-    // rows shift down one lane, lane G - 2 takes the next (all-ones) word and
-    // the sentinel restarts at 0.
-    if (l == 0) lcs += 32u - __popc(row);
-    const uint32_t up = __shfl_down(row, 1);
-    row = l == (uint32_t)(G - 2) ? 0xFFFFFFFFu : (l == (uint32_t)(G - 1) ? 0u : up);
-    vs = base + ((vs + 1u) & (uint32_t)(G - 1));
+    // B trails the band bottom of block cb + 1: when it moves up K words, the
+    // words in lane base leave (counted there), the rows shift down one lane
+    // and the top lane's words are new (all-ones) but for the sentinel; the
+    // sentinel restarts at 0 every block.
+    const int32_t Bn = (((int32_t)cb + 1 - a) >> LG) * K;
+    if (K == 1 || Bn != B) {
+      #pragma unroll
+      for (int m = 0; m < K; m++) {
+        if (l == 0) lcs += 32u - __popc(row[m]);
+        const uint32_t up = __shfl_down(row[m], 1);
+        // lane G - 2 took the top lane's words: its last is a new word where
+        // the sentinel was
+        const bool fresh = l == (uint32_t)(G - 1) || (l == (uint32_t)(G - 2) && m == K - 1);
+        row[m] = sentinel(m) ? 0u : (fresh ? 0xFFFFFFFFu : up);
+      }
+      vs = base + ((vs + 1u) & (uint32_t)(G - 1));
+      B = Bn;
+    } else {
+      #pragma unroll
+      for (int m = 0; m < K; m++)
+        if (sentinel(m)) row[m] = 0u;
+    }
     cb++;
     // At the pair's end: the LCS summed over the segment (the words that left
     // are counted in lane base).  No early exit here: a test every 8 blocks
     // cost more than the pairs it ended saved (these pairs are short).
     if (active && cb == nb) {
-      uint32_t part = (l < (uint32_t)(G - 1) ? 32u - __popc(row) : 0u) + (l == 0 ? lcs : 0u);
+      uint32_t part = l == 0 ? lcs : 0u;
+      #pragma unroll
+      for (int m = 0; m < K; m++)
+        if (!sentinel(m)) part += 32u - __popc(row[m]);
       for (int off = G / 2; off > 0; off >>= 1) part += __shfl_xor(part, off);
       if (l == 0) score[p] = pair_value(A, part, R, T);
       if (A.tally && l == 0) ran += ref_tiles(R, T, A.thr, nb);
@@ -1105,34 +1161,40 @@ int align_kernels_setup(size_t lds_per_block) {
   return kWaveKMax;
 }
 
-uint32_t seg_blocks_per_cu(int G) {
+uint32_t seg_blocks_per_cu(int G, int K) {
   // Resident single-wave blocks per CU as registers and LDS allow (8 per
   // SIMD at 60 VGPRs and 4 KiB): twice the 16 that other bins get, which
   // took 25% off a bin of 2.4M pairs at windows of 17-31 words.
-  static int per_cu[2] = {0, 0};
-  int& v = per_cu[G == 16 ? 0 : 1];
+  static int per_cu[3] = {0, 0, 0};
+  const int which = K == 2 ? 2 : G == 16 ? 0 : 1;
+  int& v = per_cu[which];
   if (!v) {
     int nb = 0;
-    const hipError_t e = G == 16
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_align_seg<16>, 64, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_align_seg<32>, 64, 0);
+    const hipError_t e =
+        which == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_align_seg<16, 1>, 64, 0)
+        : which == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_align_seg<32, 1>, 64, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_align_seg<16, 2>, 64, 0);
     v = (e == hipSuccess && nb > 0) ? std::min(nb, 32) : 16;
   }
   return (uint32_t)v;
 }
 
-void launch_align_seg(int G, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
+void launch_align_seg(int G, int K, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
                       const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work,
                       uint32_t* score, uint32_t blocks, hipStream_t stream) {
   require_lengths(a);
-  if (G == 16)
-    hipLaunchKernelGGL(k_align_seg<16>, dim3(blocks), dim3(64), 0, stream, a, pairs, list, off_ptr,
-                       count_ptr, work, score);
-  else if (G == 32)
-    hipLaunchKernelGGL(k_align_seg<32>, dim3(blocks), dim3(64), 0, stream, a, pairs, list, off_ptr,
-                       count_ptr, work, score);
+  if (G == 16 && K == 1)
+    hipLaunchKernelGGL((k_align_seg<16, 1>), dim3(blocks), dim3(64), 0, stream, a, pairs, list,
+                       off_ptr, count_ptr, work, score);
+  else if (G == 32 && K == 1)
+    hipLaunchKernelGGL((k_align_seg<32, 1>), dim3(blocks), dim3(64), 0, stream, a, pairs, list,
+                       off_ptr, count_ptr, work, score);
+  else if (G == 16 && K == 2)
+    hipLaunchKernelGGL((k_align_seg<16, 2>), dim3(blocks), dim3(64), 0, stream, a, pairs, list,
+                       off_ptr, count_ptr, work, score);
   else
-    throw HipError{NGIA_ERR_ARG, "no segmented kernel for G=" + std::to_string(G)};
+    throw HipError{NGIA_ERR_ARG, "no segmented kernel for G=" + std::to_string(G) +
+                                     ", K=" + std::to_string(K)};
   NGIA_HIP(hipGetLastError());
 }
 

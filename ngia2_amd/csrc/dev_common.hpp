@@ -194,12 +194,13 @@ void launch_align_win(int W, const AlignArgs& a, const uint2* pairs, const uint3
 void launch_align_wave(int K, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
                        const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work, uint32_t* score,
                        uint32_t blocks, hipStream_t stream);
-// 64 / G pairs per wavefront, one per G-lane segment (G = 16, 32), for band
-// windows of at most G - 1 words; any alphabet (the 16-code table).
+// 64 / G pairs per wavefront, one per G-lane segment, K words per lane
+// ((G, K) = (16, 1), (32, 1) or (16, 2)), for band windows of at most
+// K (G - 1) words; any alphabet (the 16-code table).
 size_t seg_lds_bytes();
-// Persistent blocks per CU for k_align_seg<G> (from the kernel's occupancy).
-uint32_t seg_blocks_per_cu(int G);
-void launch_align_seg(int G, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
+// Persistent blocks per CU for k_align_seg<G, K> (from the kernel's occupancy).
+uint32_t seg_blocks_per_cu(int G, int K = 1);
+void launch_align_seg(int G, int K, const AlignArgs& a, const uint2* pairs, const uint32_t* list,
                       const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work,
                       uint32_t* score, uint32_t blocks, hipStream_t stream);
 // Raise the dynamic-LDS limit of the kernels that need more than 64 KiB;

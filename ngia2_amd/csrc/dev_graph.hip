@@ -338,6 +338,7 @@ __device__ inline uint32_t pair_bin(uint32_t R, uint32_t T, float thr, uint32_t 
   const uint32_t a = bd.lsft / 32, c = (bd.rsft + 31) / 32, W = a + c;
   *tiles = ref_tiles(R, T, thr, (T + 31) / 32);
   if (W <= (uint32_t)kWinMax) return W;  // W >= 2 whenever T > 0
+  if (W <= 30) return kBinSeg16x2;
   if (W <= 31) return kBinSeg32;
   for (uint32_t k = 0, K = 1; K <= kmax; k++, K <<= 1)
     if (W + K - 1 <= 64u * (uint32_t)K) return kBinWave0 + k;
@@ -379,7 +380,8 @@ __global__ void k_bin_count(const uint2* __restrict__ pairs, const uint64_t* __r
 // of bin_cnt; also resets the scatter cursors.
 __global__ void k_bin_offsets(uint32_t* __restrict__ bin_cnt, uint32_t* __restrict__ bin_off,
                               uint32_t* __restrict__ cursor, uint32_t* __restrict__ remap,
-                              unsigned long long* __restrict__ bin_tiles, RouteTable route) {
+                              unsigned long long* __restrict__ bin_tiles, RouteTable route,
+                              uint32_t seg2_min) {
   if (threadIdx.x == 0) {
     for (int b = 0; b < kBins; b++) remap[b] = (uint32_t)b;
     for (int W = 2; W <= kWinMax; W++) {
@@ -391,6 +393,18 @@ __global__ void k_bin_offsets(uint32_t* __restrict__ bin_cnt, uint32_t* __restri
       bin_tiles[to] += bin_tiles[W];
       bin_cnt[W] = 0;
       bin_tiles[W] = 0;
+    }
+    // Windows of 17-30 words: two words per lane (four pairs per wavefront)
+    // when the bin has the pairs to fill the device that way; a smaller bin
+    // (long pairs, every segment in flight at once either way) keeps one
+    // word per lane, whose per-character chain is shorter.
+    const uint32_t n2 = bin_cnt[kBinSeg16x2];
+    if (n2 && n2 < seg2_min) {
+      remap[kBinSeg16x2] = kBinSeg32;
+      bin_cnt[kBinSeg32] += n2;
+      bin_tiles[kBinSeg32] += bin_tiles[kBinSeg16x2];
+      bin_cnt[kBinSeg16x2] = 0;
+      bin_tiles[kBinSeg16x2] = 0;
     }
     uint32_t s = 0;
     for (int b = 0; b < kBins; b++) {
@@ -439,14 +453,15 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const uint64_t* __restrict_
 void bin_pairs(const uint2* pairs, const uint64_t* npairs, const uint32_t* len, float thr,
                uint32_t* bin_of, uint32_t* bin_cnt, uint32_t* bin_off, uint32_t* cursor,
                uint32_t* list, unsigned long long* bin_tiles, uint32_t* score, uint32_t kmax,
-               uint32_t* remap, const RouteTable& route, uint32_t grid, hipStream_t s) {
+               uint32_t* remap, const RouteTable& route, uint32_t seg2_min, uint32_t grid,
+               hipStream_t s) {
   // A sixteenth of the grid: every block ends with one global atomic per
   // nonempty bin, and a few bins take nearly all pairs, so those counters
   // see one atomic per block.
   hipLaunchKernelGGL(k_bin_count, dim3(std::max(grid / 16, 1u)), dim3(256), 0, s, pairs, npairs, len,
                      thr, kmax, bin_of, bin_cnt, bin_tiles, score);
   hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, bin_cnt, bin_off, cursor, remap,
-                     bin_tiles, route);
+                     bin_tiles, route, seg2_min);
   hipLaunchKernelGGL(k_bin_scatter, dim3(grid), dim3(256), 0, s, npairs, bin_of, remap, bin_off,
                      cursor, list);
   NGIA_HIP(hipGetLastError());

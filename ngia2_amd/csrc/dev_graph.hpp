@@ -7,14 +7,23 @@ namespace ngia {
 
 // Pair bins: 0 = scored 0 without alignment, 2..kWinMax = k_align_win<W>,
 // kBinWave0 + log2(K) = k_align_wave<K>, kBinPlain = the plain kernel,
-// kBinSeg16 / kBinSeg32 = k_align_seg<16> / <32> (several pairs per
-// wavefront; band windows of kWinMax+1 .. 31 words go to kBinSeg32).
+// kBinSeg16 / kBinSeg32 = k_align_seg<16, 1> / <32, 1> (several pairs per
+// wavefront, one word per lane), kBinSeg16x2 = k_align_seg<16, 2> (four pairs
+// per wavefront, two words per lane): band windows of kWinMax+1 .. 30 words
+// go to kBinSeg16x2, of 31 words to kBinSeg32.
 constexpr int kBinZero = 0;
 constexpr int kBinWave0 = kWinMax + 1;  // kWinMax+1 .. +6 for K = 1..32
 constexpr int kBinPlain = kBinWave0 + 6;
 constexpr int kBinSeg16 = kBinPlain + 1;
 constexpr int kBinSeg32 = kBinSeg16 + 1;
-constexpr int kBins = kBinSeg32 + 1;
+constexpr int kBinSeg16x2 = kBinSeg32 + 1;
+constexpr int kBins = kBinSeg16x2 + 1;
+// A kBinSeg16x2 bin of fewer pairs goes to kBinSeg32 (k_bin_offsets; the
+// seg2_min argument of bin_pairs, NGIA_SEG2_MIN): on the heavy-tailed set at
+// -i 90 its 14-28k long pairs made the step 0.3 ms longer with two words per
+// lane; the log-normal set's 0.37M and 2.3M at -i 70 and 50 took 22% and 20%
+// less time.
+constexpr uint32_t kSeg2MinPairs = 32768;
 
 // Shapes a window bin (W = 2..kWinMax) can be aligned with, chosen per bin
 // size by the autotuner: one thread per pair (k_align_win<W>), 16 or 32 lanes
@@ -71,7 +80,8 @@ void pair_emit(const uint32_t* rep, uint32_t n, uint32_t bands, const uint64_t* 
 void bin_pairs(const uint2* pairs, const uint64_t* npairs, const uint32_t* len, float thr,
                uint32_t* bin_of, uint32_t* bin_cnt, uint32_t* bin_off, uint32_t* cursor,
                uint32_t* list, unsigned long long* bin_tiles, uint32_t* score, uint32_t kmax,
-               uint32_t* remap, const RouteTable& route, uint32_t grid, hipStream_t s);
+               uint32_t* remap, const RouteTable& route, uint32_t seg2_min, uint32_t grid,
+               hipStream_t s);
 void plain_bin_jobs(const uint2* pairs, const uint32_t* list, const uint32_t* bin_off,
                     const uint32_t* bin_cnt, uint32_t* jobs, hipStream_t s);
 void plain_bin_scores(const uint32_t* list, const uint32_t* bin_off, const uint32_t* bin_cnt,

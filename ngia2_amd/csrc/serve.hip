@@ -466,7 +466,8 @@ void launch_batch(ngia_serve* sv, uint32_t k, const uint8_t* res, const uint64_t
   const uint64_t* np = (const uint64_t*)&dc->total_cand;
   bin_pairs(ln.pairs.as<uint2>(), np, sv->lengths.as<uint32_t>(), sv->thr, ln.bin_of.as<uint32_t>(),
             dc->bin_cnt, dc->bin_off, dc->cursor, ln.list.as<uint32_t>(), dc->bin_tiles,
-            ln.score.as<uint32_t>(), sv->ctx->wave_kmax, dc->remap, sv->ctx->route, grid, s);
+            ln.score.as<uint32_t>(), sv->ctx->wave_kmax, dc->remap, sv->ctx->route,
+            sv->ctx->seg2_min, grid, s);
   AlignArgs aa;
   aa.packed = sv->packed.as<uint32_t>();
   aa.word_off = sv->word_off.as<uint64_t>();
@@ -481,11 +482,12 @@ void launch_batch(ngia_serve* sv, uint32_t k, const uint8_t* res, const uint64_t
                       &dc->bin_cnt[bin], dc->queue[bin], ln.score.as<uint32_t>(),
                       std::min<uint32_t>(Q * 64, persistent_blocks(sv->ctx, wave_lds_bytes(1 << kk))), s);
   }
-  for (int G = 32; G >= 16; G -= 16) {
-    const int bin = G == 16 ? kBinSeg16 : kBinSeg32;
-    launch_align_seg(G, aa, ln.pairs.as<uint2>(), ln.list.as<uint32_t>(), &dc->bin_off[bin],
+  for (const int bin : {kBinSeg16x2, kBinSeg32, kBinSeg16}) {
+    const int G = bin == kBinSeg32 ? 32 : 16, K = bin == kBinSeg16x2 ? 2 : 1;
+    launch_align_seg(G, K, aa, ln.pairs.as<uint2>(), ln.list.as<uint32_t>(), &dc->bin_off[bin],
                      &dc->bin_cnt[bin], dc->queue[bin], ln.score.as<uint32_t>(),
-                     std::min<uint32_t>(Q * 64, sv->ctx->prop.multiProcessorCount * seg_blocks_per_cu(G)),
+                     std::min<uint32_t>(Q * 64,
+                                        sv->ctx->prop.multiProcessorCount * seg_blocks_per_cu(G, K)),
                      s);
   }
   for (int W = kWinMax; W >= 2; W--)

@@ -167,7 +167,7 @@ extern "C" int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* 
               launch_align_wave(1, aa, pairs.as<uint2>(), list.as<uint32_t>(), off, cnt, work,
                                 score.as<uint32_t>(), persistent_blocks(ctx, wave_lds_bytes(1)), s);
             else
-              launch_align_seg(sh == kShapeSeg16 ? 16 : 32, aa, pairs.as<uint2>(),
+              launch_align_seg(sh == kShapeSeg16 ? 16 : 32, 1, aa, pairs.as<uint2>(),
                                list.as<uint32_t>(), off, cnt, work, score.as<uint32_t>(),
                                ctx->prop.multiProcessorCount *
                                    seg_blocks_per_cu(sh == kShapeSeg16 ? 16 : 32), s);

@@ -411,11 +411,16 @@ def test_band_edge_pairs(ctx):
     # A second context stores LCS lengths instead of scores: the band edges
     # decide the LCS of pairs that fail the identity test, whose score is 0
     # whatever the LCS, so only the raw value checks the edges exactly.
+    # The raw context keeps windows of 17-30 words on two words per lane
+    # (NGIA_SEG2_MIN=0) whatever the bin's size; the default context sends
+    # this small set's bin of them to 32 lanes, one word per lane.
     os.environ["NGIA_RAW_LCS"] = "1"
+    os.environ["NGIA_SEG2_MIN"] = "0"
     try:
         raw_ctx = ng.Context(0)
     finally:
         del os.environ["NGIA_RAW_LCS"]
+        del os.environ["NGIA_SEG2_MIN"]
     kinds = set()
     with raw_ctx:
         for identity in (5, 30, 60, 90, 99):
@@ -435,7 +440,7 @@ def test_band_edge_pairs(ctx):
                 db.close()
     waves = {w for k, w in kinds if k == "wave"}
     assert {1, 2, 4, 8, 16, 32} <= waves, kinds
-    assert ("seg", 32) in kinds, kinds
+    assert ("seg2", 16) in kinds and ("seg", 32) in kinds, kinds
 
 
 def test_makedb_bytes(ctx, c1db, tmp_path):
@@ -601,12 +606,12 @@ def expected_run_tiles(R, T, thr, lcs, tiles, period):
     return int(tiles[-1])
 
 
-def test_early_exit_tally_matches_oracle(ctx):
+def test_early_exit_tally_matches_oracle():
     """The early exit, block for block: a tally round
     (GRAPH_BATCHED_TALLY) scores every pair as the oracle does, and each bin's
     kernel ran exactly the reference tiles the oracle's per-block LCS predicts
     -- one thread per pair tests after every W blocks, a wavefront per pair
-    after every 8, the 32-lane segments not at all -- on the band-edge groups
+    after every 8, the segmented kernels not at all -- on the band-edge groups
     (shifted copies, unrelated and low-complexity sequences fail part way)."""
     rng = np.random.default_rng(71)
     groups = band_edge_groups(rng)[:5]  # base lengths 90 .. 5000
@@ -628,11 +633,16 @@ def test_early_exit_tally_matches_oracle(ctx):
         r[~ok] = np.arange(n)[~ok]
         rows.append(r.astype(np.uint32))
     rep = np.stack(rows)
-    saved = ctx.tuning()
-    kinds = set()
+    # two words per lane for every window of 17-30 words, whatever the bin's size
+    os.environ["NGIA_SEG2_MIN"] = "0"
     try:
-        ctx.route_all(ng.SHAPE_WINDOW)
-        db = ng.Database(ctx, packed, off, lengths)
+        c2 = ng.Context(0)
+    finally:
+        del os.environ["NGIA_SEG2_MIN"]
+    kinds = set()
+    with c2:
+        c2.route_all(ng.SHAPE_WINDOW)
+        db = ng.Database(c2, packed, off, lengths)
         db.set_reps(rep, 64)
         for identity in (50, 90):
             say("early-exit tally, identity", identity, "records", n)
@@ -652,7 +662,9 @@ def test_early_exit_tally_matches_oracle(ctx):
                 W = orc.band_window(R, T, thr)
                 if W <= 16:
                     key, period = ("window", W), W
-                elif W <= 31:
+                elif W <= 30:
+                    key, period = ("seg2", 16), 0
+                elif W == 31:
                     key, period = ("seg", 32), 0
                 else:
                     K = next(K for K in (1, 2, 4, 8, 16, 32) if W + K - 1 <= 64 * K)
@@ -665,9 +677,7 @@ def test_early_exit_tally_matches_oracle(ctx):
             assert sum(b["run_tiles"] for b in bins) < st["ref_tiles"]  # some pairs ended early
             kinds |= set(k for k, _ in want)
         db.close()
-    finally:
-        ctx.set_tuning(saved)
-    assert kinds == {"window", "seg", "wave"}, kinds
+    assert {"window", "seg2", "wave"} <= kinds <= {"window", "seg", "seg2", "wave"}, kinds
 
 
 def test_segmented_kernels_raw_lcs(ctx):
@@ -711,7 +721,7 @@ def test_segmented_kernels_raw_lcs(ctx):
                 db.set_reps(rep, 64)
                 db.graph(identity, 64, ng.GRAPH_BATCHED)
                 keys, sc = db.pair_scores()
-                seg_pairs += sum(b["pairs"] for b in db.bin_stats() if b["kind"] == "seg")
+                seg_pairs += sum(b["pairs"] for b in db.bin_stats() if b["kind"] in ("seg", "seg2"))
                 jobs = np.zeros((len(keys), 3), dtype=np.uint32)
                 jobs[:, 0] = keys >> np.uint64(32)
                 jobs[:, 1] = keys & np.uint64(0xFFFFFFFF)
