@@ -67,8 +67,8 @@ def kernel_name(b):
         return "k_align_wave<%d>" % b["width"]
     if b["kind"] == "seg":
         return "k_align_seg<%d>" % b["width"]
-    if b["kind"] == "seg2":
-        return "k_align_seg<%d, 2>" % b["width"]
+    if b["kind"] in ("seg2", "seg3", "seg4"):
+        return "k_align_seg<%d, %s>" % (b["width"], b["kind"][3])
     return "k_align_plain"
 
 
@@ -317,7 +317,7 @@ def align_rooflines(m, peak):
     with early exits it can exceed the peak, as it measures skipped work too."""
     rows = []
     for b in m["bins"]:
-        if b["kind"] not in ("window", "wave", "seg", "seg2", "plain") or not b["pairs"] or b["ms"] <= 0:
+        if b["kind"] not in ("window", "wave", "seg", "seg2", "seg3", "seg4", "plain") or not b["pairs"] or b["ms"] <= 0:
             continue
         sec = b["ms"] * 1e-3
         ach = b["run_tiles"] * OPS_PER_TILE / sec / 1e12

@@ -52,7 +52,8 @@ class BinStat(C.Structure):
                 ("ref_tiles", C.c_uint64), ("ms", C.c_double), ("run_tiles", C.c_uint64)]
 
 
-BIN_KINDS = {0: "zero", 1: "window", 2: "wave", 3: "plain", 4: "seg", 5: "seg2"}
+BIN_KINDS = {0: "zero", 1: "window", 2: "wave", 3: "plain", 4: "seg", 5: "seg2", 6: "seg3",
+             7: "seg4"}
 
 SHAPE_WINDOW, SHAPE_SEG16, SHAPE_SEG32, SHAPE_WAVE = 0, 1, 2, 3
 SHAPES = ("window", "seg16", "seg32", "wave")

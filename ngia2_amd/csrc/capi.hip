@@ -222,12 +222,14 @@ void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
   auto seg = [&](int bin, hipStream_t q) {
     if (!holds(bin)) return;
     mark(bin, 0, q);
-    const int G = bin == kBinSeg32 ? 32 : 16, K = bin == kBinSeg16x2 ? 2 : 1;
+    const int G = seg_lanes(bin), K = seg_words(bin);
     launch_align_seg(G, K, args(bin), db->pairs.as<uint2>(), db->list.as<uint32_t>(),
                      &dc->bin_off[bin], &dc->bin_cnt[bin], dc->queue[bin], db->score.as<uint32_t>(),
                      ctx->prop.multiProcessorCount * seg_blocks_per_cu(G, K), q);
     mark(bin, 1, q);
   };
+  seg(kBinSeg16x4, fork ? ctx->side[2] : s);
+  seg(kBinSeg16x3, fork ? ctx->side[3] : s);
   seg(kBinSeg16x2, fork ? ctx->side[0] : s);
   seg(kBinSeg32, fork ? ctx->side[0] : s);
   seg(kBinSeg16, fork ? ctx->side[3] : s);
@@ -433,7 +435,7 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
   db->have_bins = true;
   uint32_t launches = 0;
   for (int b = 0; b < kBins; b++) {
-    const bool aligner = b == kBinPlain || b == kBinSeg16 || b == kBinSeg32 || b == kBinSeg16x2 ||
+    const bool aligner = b == kBinPlain || is_seg_bin(b) ||
                          (b >= 2 && b <= kWinMax) ||
                          (b >= kBinWave0 && b < kBinPlain && (1u << (b - kBinWave0)) <= ctx->wave_kmax);
     const bool launched = aligner && ((mask >> b) & 1u);
@@ -470,8 +472,9 @@ void bucket_rows(ngia_ctx* ctx, ngia_db* db, hipStream_t s) {
 // record against itself) is at least min_window words (64 by default, at
 // least 32) at thr, in non-increasing length order, at most kHeadMax of
 // them.  A pair (r, i) of the head then has R >= T and a window at least
-// that of (T, T): every head pair goes to a wavefront bin (or, past the
-// widest, the plain bin).  With 64 the head holds the pairs of K >= 2, the
+// that of (T, T): every head pair goes to a wavefront bin, to the segmented
+// bins of 32-60 words (a min_window below 61) or, past the widest, to the
+// plain bin.  With 64 the head holds the pairs of K >= 2, the
 // long latency-bound ones; with 32 (the K = 1 pairs too) the heavy-tailed
 // step measured the same, and the log-normal set's 8,192-record head at
 // -i 50 lost 1%; with 128, 11.6 against 11.3 ms on the heavy-tailed set.
@@ -589,6 +592,8 @@ void graph_step(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands, b
     ensure_batched_space(hd, bands);
     uint32_t mask = 0;
     for (uint32_t k = 0; (1u << k) <= ctx->wave_kmax; k++) mask |= 1u << (kBinWave0 + k);
+    // a head window below 61 words lets head pairs into the 3- and 4-word bins
+    if (ctx->head_window < 61) mask |= (1u << kBinSeg16x3) | (1u << kBinSeg16x4);
     if (head_may_need_plain(ctx, hd->max_len, thr)) {
       ensure_plain_bin_space(hd, bands);
       mask |= 1u << kBinPlain;
@@ -949,8 +954,8 @@ extern "C" int ngia_db_bin_stats(ngia_ctx* ctx, ngia_db* db, ngia_bin_stat* out,
     } else if (b == kBinPlain) {
       o.kind = NGIA_BIN_PLAIN;
       o.width = 0;
-    } else if (b == kBinSeg16x2) {
-      o.kind = NGIA_BIN_SEG2;
+    } else if (b >= kBinSeg16x2) {
+      o.kind = NGIA_BIN_SEG2 + (uint32_t)(seg_words(b) - 2);
       o.width = 16u;
     } else {
       o.kind = NGIA_BIN_SEG;

@@ -858,7 +858,7 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
 
 // ------------------------------------------------ several pairs per wavefront
 // One pair per G-lane segment (64 / G pairs per wavefront), K words per lane
-// (K = 1 or 2), for band windows of at most K (G - 1) words.  In the segment
+// (K = 1..4), for band windows of at most K (G - 1) words.  In the segment
 // of lanes [base, base + G), lane base + l holds the words B + K l + m,
 // m < K, of its pair's rep; B = K floor((cb - a) / K) trails the band bottom
 // cb - a, so the segment's registers shift down one lane every K blocks
@@ -877,6 +877,8 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
 // With K = 2 a segment of 16 lanes takes windows of up to 30 words, which
 // K = 1 needs 32 lanes for: four pairs per wavefront share each character's
 // lookahead instead of two, and a lane's two words share its mask address.
+// K = 3 and 4 (windows of up to 45 and 60 words) keep four pairs per
+// wavefront where a wavefront per pair would hold 64 words for each.
 // Each segment takes the next pair from the wave's claim on the work queue
 // (WaveClaim) when its pair ends;
 // the segments run their blocks in lockstep, one without work holding code 0
@@ -889,8 +891,9 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
                                                   uint32_t* __restrict__ work,
                                                   uint32_t* __restrict__ score) {
   static_assert(G == 16 || G == 32, "segments of 16 or 32 lanes");
-  static_assert(K == 1 || K == 2, "one or two words per lane");
-  constexpr int LG = K == 1 ? 0 : 1;
+  static_assert(K >= 1 && K <= 4 && (K == 1 || G == 16), "1-4 words per lane, 16 lanes for K > 1");
+  // floor(x / K) for either sign (x / K rounds toward zero)
+  auto fdiv = [](int32_t x) { return (x >= 0 ? x : x - (K - 1)) / K; };
   // [K][16 codes][64 slots]; code 0 stays 0
   __shared__ __align__(16) uint32_t peq[K * 16 * 64];
   const uint32_t lane = threadIdx.x;
@@ -914,7 +917,7 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
   // the sentinel: the top lane's last word
   auto sentinel = [&](int m) { return l == (uint32_t)(G - 1) && m == K - 1; };
   // the slot of word w in table w mod K
-  auto slot_of = [&](int32_t w) { return base + ((uint32_t)(w >> LG) & (uint32_t)(G - 1)); };
+  auto slot_of = [&](int32_t w) { return base + ((uint32_t)fdiv(w) & (uint32_t)(G - 1)); };
   WaveClaim q(count, 64u / G, 4u * (64u / G));
   for (;;) {
     // Segments whose pair ended take the next ones; one that finds the queue
@@ -966,10 +969,10 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
               }
             }
             lds_order();
-            B = ((-a) >> LG) * K;  // K * floor(-a / K)
+            B = fdiv(-a) * K;
             #pragma unroll
             for (int m = 0; m < K; m++) row[m] = sentinel(m) ? 0u : 0xFFFFFFFFu;
-            vs = base + ((uint32_t)((B >> LG) + (int32_t)l) & (uint32_t)(G - 1));
+            vs = base + ((uint32_t)(fdiv(B) + (int32_t)l) & (uint32_t)(G - 1));
             lcs = 0;
             cb = 0;
             c0 = tcq[0];
@@ -1047,9 +1050,9 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
       const int32_t wl = (int32_t)cb - a, we = (int32_t)cb + c;
       if (l >= 1 && l < 16) {
         if (wl >= 0 && wl < nw)
-          peq[((wl & (K - 1)) * 16 + (int32_t)l) * 64 + slot_of(wl)] = 0u;
+          peq[((wl % K) * 16 + (int32_t)l) * 64 + slot_of(wl)] = 0u;
         if (we < nw)
-          peq[((we & (K - 1)) * 16 + (int32_t)l) * 64 + slot_of(we)] = code_mask(q0, q1, q2, q3, l);
+          peq[((we % K) * 16 + (int32_t)l) * 64 + slot_of(we)] = code_mask(q0, q1, q2, q3, l);
       }
       lds_order();
       if (we + 1 < nw) {
@@ -1061,7 +1064,7 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
     // words in lane base leave (counted there), the rows shift down one lane
     // and the top lane's words are new (all-ones) but for the sentinel; the
     // sentinel restarts at 0 every block.
-    const int32_t Bn = (((int32_t)cb + 1 - a) >> LG) * K;
+    const int32_t Bn = fdiv((int32_t)cb + 1 - a) * K;
     if (K == 1 || Bn != B) {
       #pragma unroll
       for (int m = 0; m < K; m++) {
@@ -1161,19 +1164,31 @@ int align_kernels_setup(size_t lds_per_block) {
   return kWaveKMax;
 }
 
+// The segmented kernel of (G, K): (16, 1), (32, 1), (16, 2), (16, 3) or (16, 4).
+using SegKernel = void (*)(AlignArgs, const uint2*, const uint32_t*, const uint32_t*,
+                           const uint32_t*, uint32_t*, uint32_t*);
+static int seg_index(int G, int K) {
+  if (G == 16 && K >= 1 && K <= 4) return K - 1;
+  if (G == 32 && K == 1) return 4;
+  throw HipError{NGIA_ERR_ARG,
+                 "no segmented kernel for G=" + std::to_string(G) + ", K=" + std::to_string(K)};
+}
+static SegKernel seg_kernel(int G, int K) {
+  static const SegKernel k[5] = {k_align_seg<16, 1>, k_align_seg<16, 2>, k_align_seg<16, 3>,
+                                 k_align_seg<16, 4>, k_align_seg<32, 1>};
+  return k[seg_index(G, K)];
+}
+
 uint32_t seg_blocks_per_cu(int G, int K) {
   // Resident single-wave blocks per CU as registers and LDS allow (8 per
   // SIMD at 60 VGPRs and 4 KiB): twice the 16 that other bins get, which
   // took 25% off a bin of 2.4M pairs at windows of 17-31 words.
-  static int per_cu[3] = {0, 0, 0};
-  const int which = K == 2 ? 2 : G == 16 ? 0 : 1;
-  int& v = per_cu[which];
+  static int per_cu[5] = {0, 0, 0, 0, 0};
+  int& v = per_cu[seg_index(G, K)];
   if (!v) {
     int nb = 0;
     const hipError_t e =
-        which == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_align_seg<16, 1>, 64, 0)
-        : which == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_align_seg<32, 1>, 64, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_align_seg<16, 2>, 64, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, seg_kernel(G, K), 64, 0);
     v = (e == hipSuccess && nb > 0) ? std::min(nb, 32) : 16;
   }
   return (uint32_t)v;
@@ -1183,18 +1198,8 @@ void launch_align_seg(int G, int K, const AlignArgs& a, const uint2* pairs, cons
                       const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work,
                       uint32_t* score, uint32_t blocks, hipStream_t stream) {
   require_lengths(a);
-  if (G == 16 && K == 1)
-    hipLaunchKernelGGL((k_align_seg<16, 1>), dim3(blocks), dim3(64), 0, stream, a, pairs, list,
-                       off_ptr, count_ptr, work, score);
-  else if (G == 32 && K == 1)
-    hipLaunchKernelGGL((k_align_seg<32, 1>), dim3(blocks), dim3(64), 0, stream, a, pairs, list,
-                       off_ptr, count_ptr, work, score);
-  else if (G == 16 && K == 2)
-    hipLaunchKernelGGL((k_align_seg<16, 2>), dim3(blocks), dim3(64), 0, stream, a, pairs, list,
-                       off_ptr, count_ptr, work, score);
-  else
-    throw HipError{NGIA_ERR_ARG, "no segmented kernel for G=" + std::to_string(G) +
-                                     ", K=" + std::to_string(K)};
+  hipLaunchKernelGGL(seg_kernel(G, K), dim3(blocks), dim3(64), 0, stream, a, pairs, list, off_ptr,
+                     count_ptr, work, score);
   NGIA_HIP(hipGetLastError());
 }
 

@@ -324,10 +324,12 @@ void pair_emit(const uint32_t* rep, uint32_t n, uint32_t bands, const uint64_t* 
 }
 
 // Bin of a pair by its band window: W = a + c words.  Bins 2..kWinMax go to
-// k_align_win<W>; larger windows to k_align_wave<K> with the smallest K in
-// {1,2,4,...,32} with 64K >= W + K - 1 (the wave trails the band bottom in
-// steps of K words); T == 0 scores 0 at once (its LCS is 0, so the
-// pass test fails); anything wider goes to the plain kernel.
+// k_align_win<W>; windows of up to 60 words to the segmented bins (16 lanes
+// per pair with 2, 3 or 4 words per lane, or 32 lanes with one at W = 31);
+// larger windows to k_align_wave<K> with the smallest K in {1,2,4,...,32}
+// with 64K >= W + K - 1 (the wave trails the band bottom in steps of K
+// words); T == 0 scores 0 at once (its LCS is 0, so the pass test fails);
+// anything wider goes to the plain kernel.
 __device__ inline uint32_t pair_bin(uint32_t R, uint32_t T, float thr, uint32_t kmax,
                                     uint64_t* tiles) {
   if (length_rejects(R, T, thr) || T == 0) {
@@ -340,6 +342,8 @@ __device__ inline uint32_t pair_bin(uint32_t R, uint32_t T, float thr, uint32_t 
   if (W <= (uint32_t)kWinMax) return W;  // W >= 2 whenever T > 0
   if (W <= 30) return kBinSeg16x2;
   if (W <= 31) return kBinSeg32;
+  if (W <= 45) return kBinSeg16x3;
+  if (W <= 60) return kBinSeg16x4;
   for (uint32_t k = 0, K = 1; K <= kmax; k++, K <<= 1)
     if (W + K - 1 <= 64u * (uint32_t)K) return kBinWave0 + k;
   return kBinPlain;
@@ -397,15 +401,21 @@ __global__ void k_bin_offsets(uint32_t* __restrict__ bin_cnt, uint32_t* __restri
     // Windows of 17-30 words: two words per lane (four pairs per wavefront)
     // when the bin has the pairs to fill the device that way; a smaller bin
     // (long pairs, every segment in flight at once either way) keeps one
-    // word per lane, whose per-character chain is shorter.
-    const uint32_t n2 = bin_cnt[kBinSeg16x2];
-    if (n2 && n2 < seg2_min) {
-      remap[kBinSeg16x2] = kBinSeg32;
-      bin_cnt[kBinSeg32] += n2;
-      bin_tiles[kBinSeg32] += bin_tiles[kBinSeg16x2];
-      bin_cnt[kBinSeg16x2] = 0;
-      bin_tiles[kBinSeg16x2] = 0;
-    }
+    // word per lane, whose per-character chain is shorter.  Likewise windows
+    // of 32-60 words: three or four words per lane, or a wavefront per pair.
+    auto fold = [&](int from, int to) {
+      const uint32_t n2 = bin_cnt[from];
+      if (n2 && n2 < seg2_min) {
+        remap[from] = (uint32_t)to;
+        bin_cnt[to] += n2;
+        bin_tiles[to] += bin_tiles[from];
+        bin_cnt[from] = 0;
+        bin_tiles[from] = 0;
+      }
+    };
+    fold(kBinSeg16x2, kBinSeg32);
+    fold(kBinSeg16x3, kBinWave0);
+    fold(kBinSeg16x4, kBinWave0);
     uint32_t s = 0;
     for (int b = 0; b < kBins; b++) {
       bin_off[b] = s;

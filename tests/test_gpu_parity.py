@@ -411,9 +411,10 @@ def test_band_edge_pairs(ctx):
     # A second context stores LCS lengths instead of scores: the band edges
     # decide the LCS of pairs that fail the identity test, whose score is 0
     # whatever the LCS, so only the raw value checks the edges exactly.
-    # The raw context keeps windows of 17-30 words on two words per lane
-    # (NGIA_SEG2_MIN=0) whatever the bin's size; the default context sends
-    # this small set's bin of them to 32 lanes, one word per lane.
+    # The raw context keeps windows of 17-30 words on two words per lane and
+    # those of 32-60 on three or four (NGIA_SEG2_MIN=0) whatever the bin's
+    # size; the default context sends this small set's bins of them to 32
+    # lanes, one word per lane, and to a wavefront per pair.
     os.environ["NGIA_RAW_LCS"] = "1"
     os.environ["NGIA_SEG2_MIN"] = "0"
     try:
@@ -440,7 +441,7 @@ def test_band_edge_pairs(ctx):
                 db.close()
     waves = {w for k, w in kinds if k == "wave"}
     assert {1, 2, 4, 8, 16, 32} <= waves, kinds
-    assert ("seg2", 16) in kinds and ("seg", 32) in kinds, kinds
+    assert {("seg2", 16), ("seg", 32), ("seg3", 16), ("seg4", 16)} <= kinds, kinds
 
 
 def test_makedb_bytes(ctx, c1db, tmp_path):
@@ -633,7 +634,8 @@ def test_early_exit_tally_matches_oracle():
         r[~ok] = np.arange(n)[~ok]
         rows.append(r.astype(np.uint32))
     rep = np.stack(rows)
-    # two words per lane for every window of 17-30 words, whatever the bin's size
+    # two words per lane for every window of 17-30 words, three or four for
+    # 32-60, whatever the bin's size
     os.environ["NGIA_SEG2_MIN"] = "0"
     try:
         c2 = ng.Context(0)
@@ -644,7 +646,7 @@ def test_early_exit_tally_matches_oracle():
         c2.route_all(ng.SHAPE_WINDOW)
         db = ng.Database(c2, packed, off, lengths)
         db.set_reps(rep, 64)
-        for identity in (50, 90):
+        for identity in (50, 70, 90):
             say("early-exit tally, identity", identity, "records", n)
             thr = float(np.float32(identity) / np.float32(100))
             st = db.graph(identity, 64, ng.GRAPH_BATCHED_TALLY)
@@ -666,6 +668,10 @@ def test_early_exit_tally_matches_oracle():
                     key, period = ("seg2", 16), 0
                 elif W == 31:
                     key, period = ("seg", 32), 0
+                elif W <= 45:
+                    key, period = ("seg3", 16), 0
+                elif W <= 60:
+                    key, period = ("seg4", 16), 0
                 else:
                     K = next(K for K in (1, 2, 4, 8, 16, 32) if W + K - 1 <= 64 * K)
                     key, period = ("wave", K), 8
@@ -677,7 +683,8 @@ def test_early_exit_tally_matches_oracle():
             assert sum(b["run_tiles"] for b in bins) < st["ref_tiles"]  # some pairs ended early
             kinds |= set(k for k, _ in want)
         db.close()
-    assert {"window", "seg2", "wave"} <= kinds <= {"window", "seg", "seg2", "wave"}, kinds
+    assert {"window", "seg2", "seg3", "seg4", "wave"} <= kinds, kinds
+    assert kinds <= {"window", "seg", "seg2", "seg3", "seg4", "wave"}, kinds
 
 
 def test_segmented_kernels_raw_lcs(ctx):
@@ -721,7 +728,7 @@ def test_segmented_kernels_raw_lcs(ctx):
                 db.set_reps(rep, 64)
                 db.graph(identity, 64, ng.GRAPH_BATCHED)
                 keys, sc = db.pair_scores()
-                seg_pairs += sum(b["pairs"] for b in db.bin_stats() if b["kind"] in ("seg", "seg2"))
+                seg_pairs += sum(b["pairs"] for b in db.bin_stats() if b["kind"].startswith("seg"))
                 jobs = np.zeros((len(keys), 3), dtype=np.uint32)
                 jobs[:, 0] = keys >> np.uint64(32)
                 jobs[:, 1] = keys & np.uint64(0xFFFFFFFF)
