@@ -29,13 +29,14 @@ namespace {
 
 enum Op {
   kXor, kAdd, kShr, kMin, kMin3, kLshlAdd, kBitop3, kPerm, kAddCo, kAddcCo, kCmp, kMulLo,
-  kCndmask, kMovSdwa, kAddCoVcc, kCmpVcc, kOps
+  kCndmask, kMovSdwa, kAddCoVcc, kCmpVcc, kLshlAdd64, kCmpEq64, kCmpLt64, kOps
 };
 const char* kNames[kOps] = {"v_xor_b32",      "v_add_u32",     "v_lshrrev_b32", "v_min_u32",
                             "v_min3_u32",     "v_lshl_add_u32", "v_bitop3_b32",  "v_perm_b32",
                             "v_add_co_u32",   "v_addc_co_u32", "v_cmp_eq_u32",  "v_mul_lo_u32",
                             "v_cndmask_b32 (e64, SGPR mask)", "v_mov_b32_sdwa (byte 1 of dst)",
-                            "v_add_co_u32_e32 (carry to VCC)", "v_cmp_eq_u32_e32 (to VCC)"};
+                            "v_add_co_u32_e32 (carry to VCC)", "v_cmp_eq_u32_e32 (to VCC)",
+                            "v_lshl_add_u64", "v_cmp_eq_u64", "v_cmp_lt_u64"};
 
 template <int OP>
 __device__ __forceinline__ void step(uint32_t& a, uint64_t& m, uint32_t b, uint32_t c) {
@@ -61,6 +62,11 @@ __device__ __forceinline__ void step(uint32_t& a, uint64_t& m, uint32_t b, uint3
   if constexpr (OP == kMovSdwa)
     asm volatile("v_mov_b32_sdwa %0, %1 dst_sel:BYTE_1 dst_unused:UNUSED_PRESERVE src0_sel:BYTE_2"
                  : "+v"(a) : "v"(b));
+  // 64-bit forms: m is the accumulator (a VGPR pair) or the compare's mask
+  const uint64_t bb = ((uint64_t)b << 32) | c, cc = ((uint64_t)c << 32) | b;
+  if constexpr (OP == kLshlAdd64) asm volatile("v_lshl_add_u64 %0, %0, 0, %1" : "+v"(m) : "v"(bb));
+  if constexpr (OP == kCmpEq64) asm volatile("v_cmp_eq_u64 %0, %1, %2" : "=s"(m) : "v"(bb), "v"(cc));
+  if constexpr (OP == kCmpLt64) asm volatile("v_cmp_lt_u64 %0, %1, %2" : "=s"(m) : "v"(bb), "v"(cc));
 }
 
 template <int OP>
@@ -141,6 +147,9 @@ int main(int argc, char** argv) {
   run<kMovSdwa>(out, blocks, iters, prop.clockRate, simds);
   run<kAddCoVcc>(out, blocks, iters, prop.clockRate, simds);
   run<kCmpVcc>(out, blocks, iters, prop.clockRate, simds);
+  run<kLshlAdd64>(out, blocks, iters, prop.clockRate, simds);
+  run<kCmpEq64>(out, blocks, iters, prop.clockRate, simds);
+  run<kCmpLt64>(out, blocks, iters, prop.clockRate, simds);
   CHECK(hipFree(out));
   return 0;
 }
