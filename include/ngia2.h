@@ -231,14 +231,16 @@ int ngia_db_band_jobs(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t ba
  * reference aligns every band's jobs in one kernel_dynamic launch).  A pair's
  * bin follows from its band window W = lsft/32 + ceil(rsft/32) words:
  * W <= 16 -> a window bin, aligned by the shape the tuning picks for its size
- * (one thread per pair by default); W <= 30 -> 16 lanes per pair with two
- * words per lane, four pairs per wavefront; W = 31 -> 32 lanes per pair, two
- * pairs per wavefront; W <= 45 and W <= 60 -> 16 lanes per pair with three and
- * four words per lane; W <= 64*K - K + 1 -> a wavefront per pair with K words
- * per lane; wider -> the plain kernel; length-rejected or empty targets score
- * 0 without a kernel.  A bin of 17..30 words holding fewer than 32,768 pairs
- * goes to 32 lanes per pair, one of 32..60 words to a wavefront per pair
- * (NGIA_SEG2_MIN in the environment of ngia_open overrides the limit). */
+ * (one thread per pair by default); W <= 21 -> 8 lanes per pair with three
+ * words per lane, eight pairs per wavefront; W <= 30 -> 16 lanes per pair with
+ * two words per lane, four pairs per wavefront; W = 31 ->
+ * 32 lanes per pair, two pairs per wavefront; W <= 45 and W <= 60 -> 16 lanes
+ * per pair with three and four words per lane; W <= 64*K - K + 1 -> a
+ * wavefront per pair with K words per lane; wider -> the plain kernel;
+ * length-rejected or empty targets score 0 without a kernel.  A bin of 17..30
+ * words holding fewer than 32,768 pairs goes to 32 lanes per pair, one of
+ * 32..60 words to a wavefront per pair (NGIA_SEG2_MIN in the environment of
+ * ngia_open overrides the limit). */
 #define NGIA_BIN_ZERO 0u
 #define NGIA_BIN_WINDOW 1u   /* k_align_win<W>: one thread per pair; width = W */
 #define NGIA_BIN_WAVE 2u     /* k_align_wave<K>: one wavefront per pair; width = K */

@@ -230,6 +230,7 @@ void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
   };
   seg(kBinSeg16x4, fork ? ctx->side[2] : s);
   seg(kBinSeg16x3, fork ? ctx->side[3] : s);
+  seg(kBinSeg8x3, fork ? ctx->side[1] : s);
   seg(kBinSeg16x2, fork ? ctx->side[0] : s);
   seg(kBinSeg32, fork ? ctx->side[0] : s);
   seg(kBinSeg16, fork ? ctx->side[3] : s);
@@ -956,7 +957,7 @@ extern "C" int ngia_db_bin_stats(ngia_ctx* ctx, ngia_db* db, ngia_bin_stat* out,
       o.width = 0;
     } else if (b >= kBinSeg16x2) {
       o.kind = NGIA_BIN_SEG2 + (uint32_t)(seg_words(b) - 2);
-      o.width = 16u;
+      o.width = (uint32_t)seg_lanes(b);
     } else {
       o.kind = NGIA_BIN_SEG;
       o.width = b == kBinSeg16 ? 16u : 32u;

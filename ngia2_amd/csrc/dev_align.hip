@@ -858,7 +858,8 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
 
 // ------------------------------------------------ several pairs per wavefront
 // One pair per G-lane segment (64 / G pairs per wavefront), K words per lane
-// (K = 1..4), for band windows of at most K (G - 1) words.  In the segment
+// ((G, K) = (32, 1), (16, 1..4) or (8, 3)), for band windows of at most
+// K (G - 1) words.  In the segment
 // of lanes [base, base + G), lane base + l holds the words B + K l + m,
 // m < K, of its pair's rep; B = K floor((cb - a) / K) trails the band bottom
 // cb - a, so the segment's registers shift down one lane every K blocks
@@ -878,7 +879,10 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
 // K = 1 needs 32 lanes for: four pairs per wavefront share each character's
 // lookahead instead of two, and a lane's two words share its mask address.
 // K = 3 and 4 (windows of up to 45 and 60 words) keep four pairs per
-// wavefront where a wavefront per pair would hold 64 words for each.
+// wavefront where a wavefront per pair would hold 64 words for each; with
+// 8-lane segments of three words (windows of up to 21 words) eight pairs
+// share each lookahead, and a pair holds fewer words beyond its band than in
+// 16 lanes of two words.
 // Each segment takes the next pair from the wave's claim on the work queue
 // (WaveClaim) when its pair ends;
 // the segments run their blocks in lockstep, one without work holding code 0
@@ -890,8 +894,8 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
                                                   const uint32_t* __restrict__ count_ptr,
                                                   uint32_t* __restrict__ work,
                                                   uint32_t* __restrict__ score) {
-  static_assert(G == 16 || G == 32, "segments of 16 or 32 lanes");
-  static_assert(K >= 1 && K <= 4 && (K == 1 || G == 16), "1-4 words per lane, 16 lanes for K > 1");
+  static_assert((G == 32 && K == 1) || (G == 16 && K >= 1 && K <= 4) || (G == 8 && K == 3),
+                "segments of 32 lanes with one word, 16 with 1-4, 8 with 3");
   // floor(x / K) for either sign (x / K rounds toward zero)
   auto fdiv = [](int32_t x) { return (x >= 0 ? x : x - (K - 1)) / K; };
   // [K][16 codes][64 slots]; code 0 stays 0
@@ -1048,11 +1052,16 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
     // Word cb - a leaves the band, word cb + c enters it for block cb + 1.
     if (active) {
       const int32_t wl = (int32_t)cb - a, we = (int32_t)cb + c;
-      if (l >= 1 && l < 16) {
-        if (wl >= 0 && wl < nw)
-          peq[((wl % K) * 16 + (int32_t)l) * 64 + slot_of(wl)] = 0u;
-        if (we < nw)
-          peq[((we % K) * 16 + (int32_t)l) * 64 + slot_of(we)] = code_mask(q0, q1, q2, q3, l);
+      // lane l writes code l's mask (and, in 8-lane segments, code l + 8's)
+      #pragma unroll
+      for (int h = 0; h < (G < 16 ? 2 : 1); h++) {
+        const uint32_t k = l + (uint32_t)(h * G);
+        if (k >= 1 && k < 16) {
+          if (wl >= 0 && wl < nw)
+            peq[((wl % K) * 16 + (int32_t)k) * 64 + slot_of(wl)] = 0u;
+          if (we < nw)
+            peq[((we % K) * 16 + (int32_t)k) * 64 + slot_of(we)] = code_mask(q0, q1, q2, q3, k);
+        }
       }
       lds_order();
       if (we + 1 < nw) {
@@ -1164,18 +1173,19 @@ int align_kernels_setup(size_t lds_per_block) {
   return kWaveKMax;
 }
 
-// The segmented kernel of (G, K): (16, 1), (32, 1), (16, 2), (16, 3) or (16, 4).
+// The segmented kernel of (G, K): (16, 1..4), (32, 1) or (8, 3).
 using SegKernel = void (*)(AlignArgs, const uint2*, const uint32_t*, const uint32_t*,
                            const uint32_t*, uint32_t*, uint32_t*);
 static int seg_index(int G, int K) {
   if (G == 16 && K >= 1 && K <= 4) return K - 1;
   if (G == 32 && K == 1) return 4;
+  if (G == 8 && K == 3) return 5;
   throw HipError{NGIA_ERR_ARG,
                  "no segmented kernel for G=" + std::to_string(G) + ", K=" + std::to_string(K)};
 }
 static SegKernel seg_kernel(int G, int K) {
-  static const SegKernel k[5] = {k_align_seg<16, 1>, k_align_seg<16, 2>, k_align_seg<16, 3>,
-                                 k_align_seg<16, 4>, k_align_seg<32, 1>};
+  static const SegKernel k[6] = {k_align_seg<16, 1>, k_align_seg<16, 2>, k_align_seg<16, 3>,
+                                 k_align_seg<16, 4>, k_align_seg<32, 1>, k_align_seg<8, 3>};
   return k[seg_index(G, K)];
 }
 
@@ -1183,7 +1193,7 @@ uint32_t seg_blocks_per_cu(int G, int K) {
   // Resident single-wave blocks per CU as registers and LDS allow (8 per
   // SIMD at 60 VGPRs and 4 KiB): twice the 16 that other bins get, which
   // took 25% off a bin of 2.4M pairs at windows of 17-31 words.
-  static int per_cu[5] = {0, 0, 0, 0, 0};
+  static int per_cu[6] = {0, 0, 0, 0, 0, 0};
   int& v = per_cu[seg_index(G, K)];
   if (!v) {
     int nb = 0;

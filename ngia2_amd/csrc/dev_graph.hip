@@ -324,8 +324,9 @@ void pair_emit(const uint32_t* rep, uint32_t n, uint32_t bands, const uint64_t* 
 }
 
 // Bin of a pair by its band window: W = a + c words.  Bins 2..kWinMax go to
-// k_align_win<W>; windows of up to 60 words to the segmented bins (16 lanes
-// per pair with 2, 3 or 4 words per lane, or 32 lanes with one at W = 31);
+// k_align_win<W>; windows of up to 60 words to the segmented bins (8 lanes
+// per pair with 3 words per lane up to 21 words, 16 lanes with 2 at 22-30,
+// 32 lanes with one at 31, 16 lanes with 3 or 4 at 32-60);
 // larger windows to k_align_wave<K> with the smallest K in {1,2,4,...,32}
 // with 64K >= W + K - 1 (the wave trails the band bottom in steps of K
 // words); T == 0 scores 0 at once (its LCS is 0, so the pass test fails);
@@ -340,6 +341,7 @@ __device__ inline uint32_t pair_bin(uint32_t R, uint32_t T, float thr, uint32_t 
   const uint32_t a = bd.lsft / 32, c = (bd.rsft + 31) / 32, W = a + c;
   *tiles = ref_tiles(R, T, thr, (T + 31) / 32);
   if (W <= (uint32_t)kWinMax) return W;  // W >= 2 whenever T > 0
+  if (W <= 21) return kBinSeg8x3;
   if (W <= 30) return kBinSeg16x2;
   if (W <= 31) return kBinSeg32;
   if (W <= 45) return kBinSeg16x3;
@@ -398,11 +400,12 @@ __global__ void k_bin_offsets(uint32_t* __restrict__ bin_cnt, uint32_t* __restri
       bin_cnt[W] = 0;
       bin_tiles[W] = 0;
     }
-    // Windows of 17-30 words: two words per lane (four pairs per wavefront)
-    // when the bin has the pairs to fill the device that way; a smaller bin
-    // (long pairs, every segment in flight at once either way) keeps one
-    // word per lane, whose per-character chain is shorter.  Likewise windows
-    // of 32-60 words: three or four words per lane, or a wavefront per pair.
+    // Windows of 17-30 words: several words per lane (four or eight pairs
+    // per wavefront) when the bin has the pairs to fill the device that way;
+    // a smaller bin (long pairs, every segment in flight at once either way)
+    // keeps one word per lane, whose per-character chain is shorter.
+    // Likewise windows of 32-60 words: three or four words per lane, or a
+    // wavefront per pair.
     auto fold = [&](int from, int to) {
       const uint32_t n2 = bin_cnt[from];
       if (n2 && n2 < seg2_min) {
@@ -413,6 +416,7 @@ __global__ void k_bin_offsets(uint32_t* __restrict__ bin_cnt, uint32_t* __restri
         bin_tiles[from] = 0;
       }
     };
+    fold(kBinSeg8x3, kBinSeg32);
     fold(kBinSeg16x2, kBinSeg32);
     fold(kBinSeg16x3, kBinWave0);
     fold(kBinSeg16x4, kBinWave0);

@@ -9,9 +9,11 @@ namespace ngia {
 // kBinWave0 + log2(K) = k_align_wave<K>, kBinPlain = the plain kernel,
 // kBinSeg16 / kBinSeg32 = k_align_seg<16, 1> / <32, 1> (several pairs per
 // wavefront, one word per lane), kBinSeg16x2..x4 = k_align_seg<16, 2..4>
-// (four pairs per wavefront, 2-4 words per lane): band windows of
-// kWinMax+1 .. 30 words go to kBinSeg16x2, of 31 words to kBinSeg32, of
-// 32..45 to kBinSeg16x3 and of 46..60 to kBinSeg16x4.
+// (four pairs per wavefront, 2-4 words per lane), kBinSeg8x3 =
+// k_align_seg<8, 3> (eight pairs per wavefront): band windows of
+// kWinMax+1 .. 21 words go to kBinSeg8x3, of 22..30 to kBinSeg16x2, of 31
+// words to kBinSeg32, of 32..45 to kBinSeg16x3 and of 46..60 to
+// kBinSeg16x4.
 constexpr int kBinZero = 0;
 constexpr int kBinWave0 = kWinMax + 1;  // kWinMax+1 .. +6 for K = 1..32
 constexpr int kBinPlain = kBinWave0 + 6;
@@ -20,18 +22,22 @@ constexpr int kBinSeg32 = kBinSeg16 + 1;
 constexpr int kBinSeg16x2 = kBinSeg32 + 1;
 constexpr int kBinSeg16x3 = kBinSeg16x2 + 1;
 constexpr int kBinSeg16x4 = kBinSeg16x3 + 1;
-constexpr int kBins = kBinSeg16x4 + 1;
+constexpr int kBinSeg8x3 = kBinSeg16x4 + 1;
+constexpr int kBins = kBinSeg8x3 + 1;
 static_assert(kBins <= 32, "bin masks are 32-bit");
-// A kBinSeg16x2 bin of fewer pairs goes to kBinSeg32, a kBinSeg16x3 or x4
-// bin to k_align_wave<1> (k_bin_offsets; the seg2_min argument of bin_pairs,
-// NGIA_SEG2_MIN): on the heavy-tailed set at -i 90 its 14-28k long pairs
-// made the step 0.3 ms longer with two words per lane; the log-normal set's
-// 0.37M and 2.3M at -i 70 and 50 took 22% and 20% less time.
+// A kBinSeg16x2 or kBinSeg8x3 bin of fewer pairs goes to kBinSeg32, a
+// kBinSeg16x3 or x4 bin to k_align_wave<1> (k_bin_offsets; the seg2_min
+// argument of bin_pairs, NGIA_SEG2_MIN): on the heavy-tailed set at -i 90 its
+// 14-28k long pairs made the step 0.3 ms longer with two words per lane; the
+// log-normal set's 0.37M and 2.3M at -i 70 and 50 took 22% and 20% less
+// time.
 constexpr uint32_t kSeg2MinPairs = 32768;
 // The segmented bins and their (lanes per pair, words per lane).
-constexpr bool is_seg_bin(int b) { return b >= kBinSeg16 && b <= kBinSeg16x4; }
-constexpr int seg_lanes(int b) { return b == kBinSeg32 ? 32 : 16; }
-constexpr int seg_words(int b) { return b >= kBinSeg16x2 ? b - kBinSeg16x2 + 2 : 1; }
+constexpr bool is_seg_bin(int b) { return b >= kBinSeg16 && b <= kBinSeg8x3; }
+constexpr int seg_lanes(int b) { return b == kBinSeg32 ? 32 : b == kBinSeg8x3 ? 8 : 16; }
+constexpr int seg_words(int b) {
+  return b == kBinSeg8x3 ? 3 : b >= kBinSeg16x2 ? b - kBinSeg16x2 + 2 : 1;
+}
 
 // Shapes a window bin (W = 2..kWinMax) can be aligned with, chosen per bin
 // size by the autotuner: one thread per pair (k_align_win<W>), 16 or 32 lanes

@@ -159,14 +159,18 @@ def test_config4_graph(ctx, config4, identity):
 def split_ctx():
     """A context that splits heads of 16 records or more, whose own band
     windows span 32 words or more (the samples here hold a few dozen to a few
-    hundred such records)."""
+    hundred such records), and aligns windows of 17-60 words with two to four
+    words per lane whatever a bin's size: the head round's pairs of 32-60
+    words then go to the segmented bins too."""
     os.environ["NGIA_HEAD_MIN"] = "16"
     os.environ["NGIA_HEAD_WINDOW"] = "32"
+    os.environ["NGIA_SEG2_MIN"] = "0"
     try:
         c = ng.Context(0)
     finally:
         del os.environ["NGIA_HEAD_MIN"]
         del os.environ["NGIA_HEAD_WINDOW"]
+        del os.environ["NGIA_SEG2_MIN"]
     yield c
     c.close()
 

@@ -411,10 +411,11 @@ def test_band_edge_pairs(ctx):
     # A second context stores LCS lengths instead of scores: the band edges
     # decide the LCS of pairs that fail the identity test, whose score is 0
     # whatever the LCS, so only the raw value checks the edges exactly.
-    # The raw context keeps windows of 17-30 words on two words per lane and
-    # those of 32-60 on three or four (NGIA_SEG2_MIN=0) whatever the bin's
-    # size; the default context sends this small set's bins of them to 32
-    # lanes, one word per lane, and to a wavefront per pair.
+    # The raw context keeps windows of 17-30 words on 8 lanes of three words
+    # or 16 of two, and those of 32-60 on 16 lanes of three or four
+    # (NGIA_SEG2_MIN=0), whatever the bin's size; the default context sends
+    # this small set's bins of them to 32 lanes, one word per lane, and to a
+    # wavefront per pair.
     os.environ["NGIA_RAW_LCS"] = "1"
     os.environ["NGIA_SEG2_MIN"] = "0"
     try:
@@ -441,7 +442,7 @@ def test_band_edge_pairs(ctx):
                 db.close()
     waves = {w for k, w in kinds if k == "wave"}
     assert {1, 2, 4, 8, 16, 32} <= waves, kinds
-    assert {("seg2", 16), ("seg", 32), ("seg3", 16), ("seg4", 16)} <= kinds, kinds
+    assert {("seg3", 8), ("seg2", 16), ("seg", 32), ("seg3", 16), ("seg4", 16)} <= kinds, kinds
 
 
 def test_makedb_bytes(ctx, c1db, tmp_path):
@@ -634,8 +635,8 @@ def test_early_exit_tally_matches_oracle():
         r[~ok] = np.arange(n)[~ok]
         rows.append(r.astype(np.uint32))
     rep = np.stack(rows)
-    # two words per lane for every window of 17-30 words, three or four for
-    # 32-60, whatever the bin's size
+    # several words per lane for every window of 17-30 and 32-60 words,
+    # whatever the bin's size
     os.environ["NGIA_SEG2_MIN"] = "0"
     try:
         c2 = ng.Context(0)
@@ -664,6 +665,8 @@ def test_early_exit_tally_matches_oracle():
                 W = orc.band_window(R, T, thr)
                 if W <= 16:
                     key, period = ("window", W), W
+                elif W <= 21:
+                    key, period = ("seg3", 8), 0
                 elif W <= 30:
                     key, period = ("seg2", 16), 0
                 elif W == 31:
