@@ -516,8 +516,10 @@ def main():
                            [round(p["ms"].get(x, 0.0), 4) for x in ng.SHAPES] for p in pts],
                 "points_columns": ["width", "dna", "pairs"] + ["%s_ms" % x for x in ng.SHAPES],
                 "fixed": "K per wavefront bin is the smallest that holds the band window "
-                         "(64K >= W + K - 1); windows of 17-31 words take 32 lanes per pair; "
-                         "persistent grids are as many blocks as the CUs hold (LDS-bound)"}
+                         "(64K >= W + K - 1); windows of 17-21 words take 8 lanes of three words "
+                         "per pair, 22-30 16 lanes of two, 31 32 lanes of one, 32-45 and 46-60 16 "
+                         "lanes of three and four (bins under 32,768 pairs: 32 lanes of one, or a "
+                         "wavefront per pair); persistent grids are as many blocks as the CUs hold"}
         log("autotune: %.2f s, shapes %s" % (tune["seconds"], {
             W: sorted(set(v.values())) for W, v in tune["shapes"].items()}))
     traffic = load_traffic()
