@@ -632,6 +632,7 @@ def main():
                              "queries_per_s": mid["queries_per_s"]}
 
     c1_gpu = {}
+    tuned_at = None if args.no_tune else args.identity
     for spec in [x for x in args.also_configs.split(",") if x]:
         cfg, idents = spec.split(":")
         cfg = int(cfg)
@@ -643,11 +644,16 @@ def main():
         o2 = {"workload": "%d %s in %d families, %s" % (d2.n, kind, p2.n_families, lens),
               "residues": int(d2.lengths.astype(np.int64).sum()), "by_identity": {}}
         for ident in (int(v) for v in idents.split("/")):
+            if tuned_at is not None and tuned_at != ident:
+                # the window-bin routing is tuned at the identity measured
+                ctx.autotune(ident)
+                tuned_at = ident
             plain = 0 if args.no_plain else (3 if cfg != 3 else 1)
             m2 = measure(ng, np, ctx, info, d2, ident, args.steps, args.warmup, plain,
                          "config %d -i %d" % (cfg, ident), split=not args.no_split)
             rows2, phase2 = align_rooflines(m2, peak)
-            r = {"identity": ident, "value": round(m2["value"], 1),
+            r = {"identity": ident, "routing_tuned_at_identity": tuned_at,
+                 "value": round(m2["value"], 1),
                  "ms_per_step": round(m2["ms_per_step"], 3),
                  "stage_ms": {"sketch": round(m2["sketch_ms"], 3),
                               "bucket": round(m2["bucket_ms"], 3),
