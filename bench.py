@@ -513,13 +513,23 @@ def main():
                 "class_size": sizes, "shapes": shapes(t["shape"]),
                 "shapes_compact": shapes(t["shape_compact"]),
                 "points": [[p["width"], p["dna"], p["pairs"]] +
-                           [round(p["ms"].get(x, 0.0), 4) for x in ng.SHAPES] for p in pts],
+                           [round(p["ms"].get(x, 0.0), 4) for x in ng.SHAPES] for p in pts
+                           if p["width"] <= 16],
                 "points_columns": ["width", "dna", "pairs"] + ["%s_ms" % x for x in ng.SHAPES],
+                # windows of 17..60 words, by group: several words per lane or one
+                "long_windows": {"17-21": "8 lanes x 3 words, or 32 lanes x 1",
+                                 "22-30": "16 lanes x 2 words, or 32 lanes x 1",
+                                 "32-45": "16 lanes x 3 words, or a wavefront per pair",
+                                 "46-60": "16 lanes x 4 words, or a wavefront per pair"},
+                "long_shapes": {g: {str(sizes[k]): ng.LONG_SHAPES[t["long"][i][k]]
+                                    for k in range(len(sizes))}
+                                for i, g in enumerate(("17-21", "22-30", "32-45", "46-60"))},
+                "long_points": [[p["width"], p["pairs"], round(p["ms"]["multi"], 4),
+                                 round(p["ms"]["single"], 4)] for p in pts if p["width"] > 16],
+                "long_points_columns": ["tuned_at_window", "pairs", "multi_ms", "single_ms"],
                 "fixed": "K per wavefront bin is the smallest that holds the band window "
-                         "(64K >= W + K - 1); windows of 17-21 words take 8 lanes of three words "
-                         "per pair, 22-30 16 lanes of two, 31 32 lanes of one, 32-45 and 46-60 16 "
-                         "lanes of three and four (bins under 32,768 pairs: 32 lanes of one, or a "
-                         "wavefront per pair); persistent grids are as many blocks as the CUs hold"}
+                         "(64K >= W + K - 1); window 31 takes 32 lanes of one word; persistent "
+                         "grids are as many blocks as the CUs hold"}
         log("autotune: %.2f s, shapes %s" % (tune["seconds"], {
             W: sorted(set(v.values())) for W, v in tune["shapes"].items()}))
     traffic = load_traffic()

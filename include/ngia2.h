@@ -275,6 +275,15 @@ int ngia_db_bin_stats(ngia_ctx* ctx, ngia_db* db, ngia_bin_stat* out, uint32_t c
 #define NGIA_SHAPE_SEG32 2u   /* k_align_seg<32>: two pairs per wavefront */
 #define NGIA_SHAPE_WAVE 3u    /* k_align_wave<1> */
 #define NGIA_TUNE_CLASSES 8
+/* Windows of 17..60 words come in four groups, each aligned with several
+ * words per lane or one: 17..21 (eight 8-lane segments of three words, or 32
+ * lanes of one), 22..30 (four 16-lane segments of two words, or 32 lanes),
+ * 32..45 and 46..60 (four 16-lane segments of three and four words, or a
+ * wavefront per pair). */
+#define NGIA_LONG_GROUPS 4
+#define NGIA_LONG_DEFAULT 0u /* several words per lane from 32,768 pairs (NGIA_SEG2_MIN) */
+#define NGIA_LONG_MULTI 1u   /* several words per lane */
+#define NGIA_LONG_SINGLE 2u  /* one word per lane, or a wavefront per pair */
 typedef struct {
   /* window bin W holding c pairs takes shape[W][k], k the last size class with
    * class_size[k] <= c (class 0 below class_size[0]); class sizes ascend, 0
@@ -285,6 +294,9 @@ typedef struct {
   /* the same for databases whose residues use at most five codes (DNA), whose
    * window kernels index a compact table */
   uint32_t shape_compact[17][NGIA_TUNE_CLASSES];
+  /* group g of windows of 17..60 words holding c pairs: NGIA_LONG_* by the
+   * same size classes (both alphabets) */
+  uint32_t long_shape[NGIA_LONG_GROUPS][NGIA_TUNE_CLASSES];
 } ngia_tuning;
 int ngia_get_tuning(ngia_ctx* ctx, ngia_tuning* out);
 /* NGIA_ERR_ARG for an unknown shape or NGIA_SHAPE_SEG16 at W = 16. */
@@ -294,13 +306,17 @@ typedef struct {
   uint32_t dna;      /* 1: the DNA pass (compact table), 0: proteins */
   uint32_t length;   /* rep length of the synthetic pairs (targets are up to 64 shorter) */
   uint32_t pairs;    /* pairs in the timed bin */
-  double ms[4];      /* device time (HIP events) per NGIA_SHAPE_*, 0 where not applicable */
+  double ms[4];      /* device time (HIP events) per NGIA_SHAPE_*, 0 where not applicable;
+                      * for W > 16 (a window of a group of 17..60 words): ms[0] several
+                      * words per lane, ms[1] one word per lane or a wavefront per pair */
 } ngia_tune_point;
 /* Time every shape on synthetic pairs of every window width W at `identity`
  * (random reps with targets of nearly the same length, chosen so that the
- * band window is exactly W; proteins, then DNA), at bins of 16..65536 pairs,
- * and install the fastest shape per (W, size class) in the context's tuning.
- * points (may be NULL) receive the measurements. */
+ * band window is exactly W; proteins, then DNA), at bins of 16..262144 pairs,
+ * and install the fastest shape per (W, size class) in the context's tuning;
+ * likewise both shapes of each group of 17..60-word windows, on proteins at
+ * the windows 19, 26, 38 and 53.  points (may be NULL) receive the
+ * measurements. */
 int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* points, uint32_t cap,
                   uint32_t* n_points);
 

@@ -13,7 +13,8 @@ import numpy as np
 from . import _lib
 from ._lib import (BANDS, GRAPH_BATCHED, GRAPH_BATCHED_EAGER, GRAPH_BATCHED_TALLY, GRAPH_PLAIN,
                    NGIA_ERR_ARG, NGIA_ERR_IO, NGIA_ERR_NODEV, NGIA_ERR_STATE, NGIA_OK, BIN_DIR,
-                   LIB_PATH, SHAPE_WINDOW, SHAPE_SEG16, SHAPE_SEG32, SHAPE_WAVE, SHAPES)
+                   LIB_PATH, SHAPE_WINDOW, SHAPE_SEG16, SHAPE_SEG32, SHAPE_WAVE, SHAPES,
+                   LONG_DEFAULT, LONG_MULTI, LONG_SINGLE, LONG_SHAPES, LONG_WINDOWS)
 
 __all__ = ["NgiaError", "Context", "Database", "PackedDB", "Serve", "Queries", "gen_config",
            "gen_fasta", "gen_packed", "gen_queries", "read_db", "leiden", "resolution",
@@ -81,14 +82,16 @@ class Context:
 
     def tuning(self):
         """Shapes of the window bins by size (ngia_get_tuning): a dict with
-        class_size (8 ascending bin sizes, 0 ends the list) and shape /
+        class_size (8 ascending bin sizes, 0 ends the list), shape /
         shape_compact, 17 x 8 lists of SHAPE_* for window width W and size
-        class k (the 16-code table and compact alphabets)."""
+        class k (the 16-code table and compact alphabets), and long, 4 x 8
+        lists of LONG_* for the groups of 17..60-word windows."""
         t = _lib.Tuning()
         _check(_lib.lib().ngia_get_tuning(self.h, C.byref(t)), self.h)
         return {"class_size": list(t.class_size),
                 "shape": [list(t.shape[W]) for W in range(17)],
-                "shape_compact": [list(t.shape_compact[W]) for W in range(17)]}
+                "shape_compact": [list(t.shape_compact[W]) for W in range(17)],
+                "long": [list(t.long_shape[g]) for g in range(_lib.LONG_GROUPS)]}
 
     def set_tuning(self, tuning):
         t = _lib.Tuning()
@@ -98,6 +101,9 @@ class Context:
             for W, row in enumerate(tuning.get(name, [])):
                 for k, v in enumerate(row):
                     getattr(t, name)[W][k] = int(v)
+        for g, row in enumerate(tuning.get("long", [])):
+            for k, v in enumerate(row):
+                t.long_shape[g][k] = int(v)
         _check(_lib.lib().ngia_set_tuning(self.h, C.byref(t)), self.h)
 
     def route_all(self, shape):
@@ -115,8 +121,11 @@ class Context:
         pts = (_lib.TunePoint * cap)()
         n = C.c_uint32()
         _check(_lib.lib().ngia_autotune(self.h, identity, pts, cap, C.byref(n)), self.h)
+        # points of the 17..60-word groups (width > 16) time two shapes:
+        # several words per lane (ms[0]) and one (ms[1])
         return [{"width": p.width, "dna": p.dna, "length": p.length, "pairs": p.pairs,
-                 "ms": {_lib.SHAPES[k]: p.ms[k] for k in range(4) if p.ms[k] > 0}}
+                 "ms": ({_lib.SHAPES[k]: p.ms[k] for k in range(4) if p.ms[k] > 0} if p.width <= 16
+                        else {"multi": p.ms[0], "single": p.ms[1]})}
                 for p in pts[:min(n.value, cap)]]
 
     def __enter__(self):

@@ -58,12 +58,18 @@ BIN_KINDS = {0: "zero", 1: "window", 2: "wave", 3: "plain", 4: "seg", 5: "seg2",
 SHAPE_WINDOW, SHAPE_SEG16, SHAPE_SEG32, SHAPE_WAVE = 0, 1, 2, 3
 SHAPES = ("window", "seg16", "seg32", "wave")
 TUNE_CLASSES = 8
+# groups of 17..60-word windows (17-21, 22-30, 32-45, 46-60) and their shapes
+LONG_GROUPS = 4
+LONG_WINDOWS = (19, 26, 38, 53)  # the window each group is tuned at
+LONG_DEFAULT, LONG_MULTI, LONG_SINGLE = 0, 1, 2
+LONG_SHAPES = ("default", "multi", "single")
 
 
 class Tuning(C.Structure):
     _fields_ = [("class_size", C.c_uint32 * TUNE_CLASSES),
                 ("shape", (C.c_uint32 * TUNE_CLASSES) * 17),
-                ("shape_compact", (C.c_uint32 * TUNE_CLASSES) * 17)]
+                ("shape_compact", (C.c_uint32 * TUNE_CLASSES) * 17),
+                ("long_shape", (C.c_uint32 * TUNE_CLASSES) * LONG_GROUPS)]
 
 
 class TunePoint(C.Structure):

@@ -346,6 +346,8 @@ void graph_batched(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t bands
     for (uint32_t w : rt.class_size) key.push_back(w);
     for (int W = 0; W <= kWinMax; W++)
       for (int k = 0; k < kTuneClasses; k++) key.push_back(rt.shape[W][k]);
+    for (int g = 0; g < kLongGroups; g++)
+      for (int k = 0; k < kTuneClasses; k++) key.push_back(rt.lng[g][k]);
   }
   key.push_back(db->t0);
   key.push_back(ctx->seg2_min);

@@ -406,9 +406,12 @@ __global__ void k_bin_offsets(uint32_t* __restrict__ bin_cnt, uint32_t* __restri
     // keeps one word per lane, whose per-character chain is shorter.
     // Likewise windows of 32-60 words: three or four words per lane, or a
     // wavefront per pair.
-    auto fold = [&](int from, int to) {
+    // The tuned table decides by the bin's size class, or leaves it to the
+    // size limit seg2_min (NGIA_LONG_DEFAULT).
+    auto fold = [&](int from, int to, int group) {
       const uint32_t n2 = bin_cnt[from];
-      if (n2 && n2 < seg2_min) {
+      const uint32_t choice = route.lng[group][route_class(route, n2)];
+      if (n2 && (choice == NGIA_LONG_SINGLE || (choice == NGIA_LONG_DEFAULT && n2 < seg2_min))) {
         remap[from] = (uint32_t)to;
         bin_cnt[to] += n2;
         bin_tiles[to] += bin_tiles[from];
@@ -416,10 +419,10 @@ __global__ void k_bin_offsets(uint32_t* __restrict__ bin_cnt, uint32_t* __restri
         bin_tiles[from] = 0;
       }
     };
-    fold(kBinSeg8x3, kBinSeg32);
-    fold(kBinSeg16x2, kBinSeg32);
-    fold(kBinSeg16x3, kBinWave0);
-    fold(kBinSeg16x4, kBinWave0);
+    fold(kBinSeg8x3, kBinSeg32, 0);
+    fold(kBinSeg16x2, kBinSeg32, 1);
+    fold(kBinSeg16x3, kBinWave0, 2);
+    fold(kBinSeg16x4, kBinWave0, 3);
     uint32_t s = 0;
     for (int b = 0; b < kBins; b++) {
       bin_off[b] = s;

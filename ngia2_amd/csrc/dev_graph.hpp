@@ -50,16 +50,25 @@ constexpr int kTuneClasses = NGIA_TUNE_CLASSES;
 // Window bin W holding c pairs is aligned with shape[W][k], k the last size
 // class with class_size[k] <= c (class 0 below class_size[0]).  A zeroed table
 // keeps every window bin on k_align_win<W>.
+// Groups of 17..60-word windows: lng[g][k] = NGIA_LONG_* for group g
+// (17..21, 22..30, 32..45, 46..60 words) in size class k.
+constexpr int kLongGroups = NGIA_LONG_GROUPS;
 struct RouteTable {
   uint32_t class_size[kTuneClasses];
   uint8_t shape[kWinMax + 1][kTuneClasses];
+  uint8_t lng[kLongGroups][kTuneClasses];
 };
-
-// The bin that aligns window bin W's c pairs under the routing table.
-__host__ __device__ inline uint32_t route_window_bin(const RouteTable& rt, uint32_t W, uint32_t c) {
+// Size class of a bin of c pairs.
+__host__ __device__ inline uint32_t route_class(const RouteTable& rt, uint32_t c) {
   uint32_t k = 0;
   for (uint32_t j = 1; j < (uint32_t)kTuneClasses; j++)
     if (rt.class_size[j] && rt.class_size[j] <= c) k = j;
+  return k;
+}
+
+// The bin that aligns window bin W's c pairs under the routing table.
+__host__ __device__ inline uint32_t route_window_bin(const RouteTable& rt, uint32_t W, uint32_t c) {
+  const uint32_t k = route_class(rt, c);
   switch (rt.shape[W][k]) {
     case kShapeSeg16: return W <= 15 ? (uint32_t)kBinSeg16 : (uint32_t)kBinSeg32;
     case kShapeSeg32: return (uint32_t)kBinSeg32;

@@ -9,7 +9,11 @@
 // shape on synthetic pairs whose band window is exactly W (equal-length
 // random sequences) at bin sizes of 16 to 262,144 pairs, and installs the
 // fastest shape per (W, size class); k_bin_offsets then moves each window
-// bin's pairs to the bin of its shape on the device.
+// bin's pairs to the bin of its shape on the device.  Windows of 17..60
+// words come in four groups, each with a shape of several words per lane
+// (eight or four pairs per wavefront) and one of one word per lane (32 lanes,
+// or a wavefront per pair); the tuner times both per group and size class
+// the same way and installs the faster.
 #include <random>
 
 #include "capi_internal.hpp"
@@ -47,6 +51,15 @@ float time_kernel(ngia_ctx* ctx, const std::function<void()>& launch, uint32_t* 
 // Whether shape k can align a window of W words.
 bool shape_fits(uint32_t k, uint32_t W) { return k != kShapeSeg16 || W <= 15; }
 
+// The groups of 17..60-word windows: the window the tuner times each at, and
+// its two shapes as (lanes, words per lane), lanes 64 meaning k_align_wave<1>.
+struct LongGroup {
+  uint32_t W;
+  int g_multi, k_multi, g_single;
+};
+constexpr LongGroup kLong[kLongGroups] = {{19, 8, 3, 32}, {26, 16, 2, 32}, {38, 16, 3, 64},
+                                          {53, 16, 4, 64}};
+
 }  // namespace
 
 extern "C" int ngia_get_tuning(ngia_ctx* ctx, ngia_tuning* out) {
@@ -58,6 +71,8 @@ extern "C" int ngia_get_tuning(ngia_ctx* ctx, ngia_tuning* out) {
       out->shape[W][k] = ctx->route.shape[W][k];
       out->shape_compact[W][k] = ctx->route_compact.shape[W][k];
     }
+  for (int g = 0; g < kLongGroups; g++)
+    for (int k = 0; k < kTuneClasses; k++) out->long_shape[g][k] = ctx->route.lng[g][k];
   return NGIA_OK;
 }
 
@@ -78,6 +93,12 @@ extern "C" int ngia_set_tuning(ngia_ctx* ctx, const ngia_tuning* t) {
           return fail(ctx, NGIA_ERR_ARG, "16-lane segments hold W <= 15");
         (tab ? rc : rt).shape[W][k] = (uint8_t)sh;
       }
+    }
+  for (int g = 0; g < kLongGroups; g++)
+    for (int k = 0; k < kTuneClasses; k++) {
+      const uint32_t sh = t->long_shape[g][k];
+      if (sh > NGIA_LONG_SINGLE) return fail(ctx, NGIA_ERR_ARG, "unknown long-window shape");
+      rt.lng[g][k] = rc.lng[g][k] = (uint8_t)sh;
     }
   ctx->route = rt;
   ctx->route_compact = rc;
@@ -111,28 +132,22 @@ extern "C" int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* 
     uint32_t np = 0;
     ngia::RouteTable route{}, route_compact{};
     for (int k = 0; k < kNumCounts; k++) route.class_size[k] = route_compact.class_size[k] = kTuneCounts[k];
-    // Two passes: proteins (the 16-code table) and DNA, whose window kernels
-    // index a compact table (at most five codes) and fit more waves per CU.
-    for (int dna = 0; dna < 2; dna++)
-    for (int W = 2; W <= kWinMax; W++) {
-      const char* alpha = dna ? "ACGT" : kAmino;
-      const uint32_t na = dna ? 4 : 20;
-      ngia::RouteTable& rt = dna ? route_compact : route;
-      // lengths R >= T passing the length test with window exactly W: the
-      // median R among T = R, R - 8, ..., R - 64
+    // Synthetic pairs whose band window is exactly W: lengths R >= T passing
+    // the length test (the median R among T = R, R - 8, ..., R - 64); pair
+    // i = (2i, 2i + 1), a random rep of length R and a target of length T,
+    // its first T residues with 5% substitutions.  Returns R, or 0 when no
+    // such lengths exist.
+    auto synth = [&](uint32_t W, const char* alpha, uint32_t na, ngia_db& db) -> uint32_t {
       std::vector<std::pair<uint32_t, uint32_t>> cand;
       for (uint32_t R = 8; R < 0xFFFF && cand.size() < 4096; R++) {
-        if (window_of(R, R, thr) > (uint32_t)W + 1) break;
+        if (window_of(R, R, thr) > W + 1) break;
         for (uint32_t d = 0; d <= 64 && d < R; d += 8) {
           const uint32_t T = R - d;
-          if (!length_rejects(R, T, thr) && window_of(R, T, thr) == (uint32_t)W) cand.push_back({R, T});
+          if (!length_rejects(R, T, thr) && window_of(R, T, thr) == W) cand.push_back({R, T});
         }
       }
-      if (cand.empty()) continue;
+      if (cand.empty()) return 0;
       const uint32_t R = cand[cand.size() / 2].first, T = cand[cand.size() / 2].second;
-      const uint32_t L = R;
-      // pair i = (2i, 2i + 1): a random rep of length R and a target of
-      // length T, its first T residues with 5% substitutions
       const uint32_t nseq = 2 * kTunePairs;
       const uint64_t rw = rec_words(R), tw = rec_words(T);
       std::vector<uint32_t> packed((rw + tw) * kTunePairs);
@@ -147,9 +162,20 @@ extern "C" int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* 
         woff[2 * i + 1] = woff[2 * i] + rw;
         pack_record(res.data(), T, packed.data() + woff[2 * i + 1]);
       }
-      ngia_db db;
       db.ctx = ctx;
       db_upload(ctx, &db, packed.data(), packed.size(), woff.data(), nullptr, nseq);
+      return R;
+    };
+    // Two passes: proteins (the 16-code table) and DNA, whose window kernels
+    // index a compact table (at most five codes) and fit more waves per CU.
+    for (int dna = 0; dna < 2; dna++)
+    for (int W = 2; W <= kWinMax; W++) {
+      const char* alpha = dna ? "ACGT" : kAmino;
+      const uint32_t na = dna ? 4 : 20;
+      ngia::RouteTable& rt = dna ? route_compact : route;
+      ngia_db db;
+      const uint32_t L = synth((uint32_t)W, alpha, na, db);
+      if (!L) continue;
       AlignArgs aa = align_args(&db, thr);
       for (int k = 0; k < kNumCounts; k++) {
         const uint32_t n = kTuneCounts[k];
@@ -181,6 +207,50 @@ extern "C" int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* 
           ngia_tune_point& pt = points[np];
           pt.width = (uint32_t)W;
           pt.dna = (uint32_t)dna;
+          pt.length = L;
+          pt.pairs = n;
+          for (int q = 0; q < 4; q++) pt.ms[q] = ms[q];
+        }
+        np++;
+      }
+    }
+    // Windows of 17..60 words: several words per lane or one, per group and
+    // size class, on proteins (these kernels use the 16-code table for any
+    // alphabet); both tables take the result.
+    for (int g = 0; g < kLongGroups; g++) {
+      const LongGroup& lg = kLong[g];
+      ngia_db db;
+      const uint32_t L = synth(lg.W, kAmino, 20, db);
+      if (!L) continue;
+      AlignArgs aa = align_args(&db, thr);
+      for (int k = 0; k < kNumCounts; k++) {
+        const uint32_t n = kTuneCounts[k];
+        const uint32_t zero = 0;
+        h2d(off, &zero, 4, s);
+        h2d(cnt, &n, 4, s);
+        double ms[4] = {0, 0, 0, 0};
+        for (int which = 0; which < 2; which++) {
+          ms[which] = time_kernel(ctx, [&] {
+            if (which == 0)
+              launch_align_seg(lg.g_multi, lg.k_multi, aa, pairs.as<uint2>(), list.as<uint32_t>(),
+                               off, cnt, work, score.as<uint32_t>(),
+                               ctx->prop.multiProcessorCount * seg_blocks_per_cu(lg.g_multi, lg.k_multi),
+                               s);
+            else if (lg.g_single == 64)
+              launch_align_wave(1, aa, pairs.as<uint2>(), list.as<uint32_t>(), off, cnt, work,
+                                score.as<uint32_t>(), persistent_blocks(ctx, wave_lds_bytes(1)), s);
+            else
+              launch_align_seg(32, 1, aa, pairs.as<uint2>(), list.as<uint32_t>(), off, cnt, work,
+                               score.as<uint32_t>(),
+                               ctx->prop.multiProcessorCount * seg_blocks_per_cu(32, 1), s);
+          }, work);
+        }
+        route.lng[g][k] = route_compact.lng[g][k] =
+            (uint8_t)(ms[1] < ms[0] ? NGIA_LONG_SINGLE : NGIA_LONG_MULTI);
+        if (points && np < cap) {
+          ngia_tune_point& pt = points[np];
+          pt.width = lg.W;
+          pt.dna = 0;
           pt.length = L;
           pt.pairs = n;
           for (int q = 0; q < 4; q++) pt.ms[q] = ms[q];

@@ -790,16 +790,80 @@ def test_autotune_installs_fastest_shapes(ctx):
     t = ctx.tuning()
     sizes = [16, 64, 256, 1024, 4096, 16384, 65536, 262144]
     assert t["class_size"] == sizes
-    widths = sorted({p["width"] for p in pts if not p["dna"]})
+    widths = sorted({p["width"] for p in pts if not p["dna"] and p["width"] <= 16})
     assert widths == list(range(3, 17))  # W = 2 cannot pass the length test
+    # the groups of 17..60-word windows, on proteins
+    assert sorted({p["width"] for p in pts if p["width"] > 16}) == list(ng.LONG_WINDOWS)
+    assert not any(p["dna"] for p in pts if p["width"] > 16)
     for p in pts:
-        assert set(p["ms"]) == ({"window", "seg32", "wave"} | ({"seg16"} if p["width"] <= 15 else set()))
         assert all(v > 0 for v in p["ms"].values())
         k = sizes.index(p["pairs"])
+        if p["width"] > 16:
+            assert set(p["ms"]) == {"multi", "single"}
+            g = ng.LONG_WINDOWS.index(p["width"])
+            want = ng.LONG_SINGLE if p["ms"]["single"] < p["ms"]["multi"] else ng.LONG_MULTI
+            assert t["long"][g][k] == want, p
+            continue
+        assert set(p["ms"]) == ({"window", "seg32", "wave"} | ({"seg16"} if p["width"] <= 15 else set()))
         table = t["shape_compact"] if p["dna"] else t["shape"]
         best = min(p["ms"], key=lambda s_: (p["ms"][s_], ng.SHAPES.index(s_)))
         assert table[p["width"]][k] == ng.SHAPES.index(best), p
     ctx.route_all(ng.SHAPE_WINDOW)
+    assert ctx.tuning()["long"] == [[ng.LONG_DEFAULT] * 8] * 4
+
+
+def test_long_window_routing_table(ctx):
+    """The tuned shapes of 17..60-word windows route their bins: with every
+    group on several words per lane, then on one word per lane (or a
+    wavefront per pair), whatever the bin's size, the band-edge groups'
+    pair scores equal the oracle's and the bins in use are those the table
+    names."""
+    rng = np.random.default_rng(29)
+    groups = band_edge_groups(rng)
+    seqs, gid = [], []
+    for k, g in enumerate(groups):
+        g = sorted(g, key=lambda s: -len(s))
+        seqs += g
+        gid += [k] * len(g)
+    recs = [orc.pack(s) for s in seqs]
+    off = np.cumsum([0] + [len(r) for r in recs[:-1]]).astype(np.uint64)
+    packed = np.concatenate(recs).astype(np.uint32)
+    lengths = np.array([len(s) for s in seqs], dtype=np.uint32)
+    n = len(seqs)
+    gid = np.array(gid)
+    rows = []
+    for b in range(64):
+        r = np.arange(n, dtype=np.int64) - 1 - b
+        ok = (r >= 0) & (gid[np.maximum(r, 0)] == gid)
+        r[~ok] = np.arange(n)[~ok]
+        rows.append(r.astype(np.uint32))
+    rep = np.stack(rows)
+    base = ctx.tuning()
+    several = {("seg3", 8), ("seg2", 16), ("seg3", 16), ("seg4", 16)}
+    try:
+        for choice in (ng.LONG_MULTI, ng.LONG_SINGLE):
+            ctx.set_tuning({"class_size": [1, 0, 0, 0, 0, 0, 0, 0],
+                            "long": [[choice] * 8] * 4})
+            kinds = set()
+            for identity in (30, 60, 90):
+                say("long-window routing", ng.LONG_SHAPES[choice], "identity", identity)
+                thr = float(np.float32(identity) / np.float32(100))
+                db = ng.Database(ctx, packed, off, lengths)
+                db.set_reps(rep, 64)
+                db.graph(identity, 64, ng.GRAPH_BATCHED)
+                keys, sc = db.pair_scores()
+                kinds |= {(b["kind"], b["width"]) for b in db.bin_stats() if b["pairs"]}
+                jobs = np.zeros((len(keys), 3), dtype=np.uint32)
+                jobs[:, 0] = keys >> np.uint64(32)
+                jobs[:, 1] = keys & np.uint64(0xFFFFFFFF)
+                np.testing.assert_array_equal(sc, orc.align_jobs(packed, off, jobs, thr)[:, 2])
+                db.close()
+            if choice == ng.LONG_MULTI:
+                assert several <= kinds, kinds
+            else:
+                assert not (several & kinds) and ("seg", 32) in kinds and ("wave", 1) in kinds, kinds
+    finally:
+        ctx.set_tuning(base)
 
 
 @pytest.fixture(scope="module")
