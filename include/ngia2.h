@@ -265,15 +265,18 @@ int ngia_db_bin_stats(ngia_ctx* ctx, ngia_db* db, ngia_bin_stat* out, uint32_t c
 /* ------------------------------------------- autotuner */
 /* No reference counterpart; the reference has one fixed shape, a thread per
  * job in blocks of 64 (buildGraph.hpp:192, 323).  A window bin (band window
- * W = 2..16 words) can be aligned in four shapes: one thread per pair (the
+ * W = 2..16 words) can be aligned in five shapes: one thread per pair (the
  * most work per instruction, but a lone pair's T x W word-characters run
- * serially on one lane), 16 or 32 lanes per pair (several pairs per
- * wavefront, one word per lane) or a wavefront per pair.  Which is fastest
- * depends on W, the bin's size, the identity (pair length) and the device. */
+ * serially on one lane, and its table of W x 4 KiB per wavefront limits the
+ * wavefronts per CU), 16 or 32 lanes per pair (several pairs per wavefront,
+ * one word per lane), 8 lanes of two words per pair, or a wavefront per pair.
+ * Which is fastest depends on W, the bin's size, the identity (pair length)
+ * and the device. */
 #define NGIA_SHAPE_WINDOW 0u  /* k_align_win<W> */
 #define NGIA_SHAPE_SEG16 1u   /* k_align_seg<16>: four pairs per wavefront (W <= 15) */
 #define NGIA_SHAPE_SEG32 2u   /* k_align_seg<32>: two pairs per wavefront */
 #define NGIA_SHAPE_WAVE 3u    /* k_align_wave<1> */
+#define NGIA_SHAPE_SEG8 4u    /* k_align_seg<8, 2>: eight pairs per wavefront (W <= 14) */
 #define NGIA_TUNE_CLASSES 8
 /* Windows of 17..60 words come in four groups, each aligned with several
  * words per lane or one: 17..21 (eight 8-lane segments of three words, or 32
@@ -299,14 +302,15 @@ typedef struct {
   uint32_t long_shape[NGIA_LONG_GROUPS][NGIA_TUNE_CLASSES];
 } ngia_tuning;
 int ngia_get_tuning(ngia_ctx* ctx, ngia_tuning* out);
-/* NGIA_ERR_ARG for an unknown shape or NGIA_SHAPE_SEG16 at W = 16. */
+/* NGIA_ERR_ARG for an unknown shape, NGIA_SHAPE_SEG16 at W = 16 or
+ * NGIA_SHAPE_SEG8 at W > 14. */
 int ngia_set_tuning(ngia_ctx* ctx, const ngia_tuning* t);
 typedef struct {
   uint32_t width;    /* band window W */
   uint32_t dna;      /* 1: the DNA pass (compact table), 0: proteins */
   uint32_t length;   /* rep length of the synthetic pairs (targets are up to 64 shorter) */
   uint32_t pairs;    /* pairs in the timed bin */
-  double ms[4];      /* device time (HIP events) per NGIA_SHAPE_*, 0 where not applicable;
+  double ms[5];      /* device time (HIP events) per NGIA_SHAPE_*, 0 where not applicable;
                       * for W > 16 (a window of a group of 17..60 words): ms[0] several
                       * words per lane, ms[1] one word per lane or a wavefront per pair */
 } ngia_tune_point;

@@ -13,7 +13,7 @@ import numpy as np
 from . import _lib
 from ._lib import (BANDS, GRAPH_BATCHED, GRAPH_BATCHED_EAGER, GRAPH_BATCHED_TALLY, GRAPH_PLAIN,
                    NGIA_ERR_ARG, NGIA_ERR_IO, NGIA_ERR_NODEV, NGIA_ERR_STATE, NGIA_OK, BIN_DIR,
-                   LIB_PATH, SHAPE_WINDOW, SHAPE_SEG16, SHAPE_SEG32, SHAPE_WAVE, SHAPES,
+                   LIB_PATH, SHAPE_WINDOW, SHAPE_SEG16, SHAPE_SEG32, SHAPE_WAVE, SHAPE_SEG8, SHAPES,
                    LONG_DEFAULT, LONG_MULTI, LONG_SINGLE, LONG_SHAPES, LONG_WINDOWS)
 
 __all__ = ["NgiaError", "Context", "Database", "PackedDB", "Serve", "Queries", "gen_config",
@@ -108,8 +108,15 @@ class Context:
 
     def route_all(self, shape):
         """Send every window bin, whatever its size, to one shape (SHAPE_SEG16
-        falls back to 32 lanes at W = 16); SHAPE_WINDOW restores the default."""
-        row = lambda W: [_lib.SHAPE_SEG32 if (shape == _lib.SHAPE_SEG16 and W == 16) else shape] * 8
+        falls back to 32 lanes at W = 16, SHAPE_SEG8 to 16 and 32 lanes at W =
+        15 and 16); SHAPE_WINDOW restores the default."""
+        def row(W):
+            sh = shape
+            if shape == _lib.SHAPE_SEG8 and W > 14:
+                sh = _lib.SHAPE_SEG16 if W == 15 else _lib.SHAPE_SEG32
+            if sh == _lib.SHAPE_SEG16 and W == 16:
+                sh = _lib.SHAPE_SEG32
+            return [sh] * 8
         self.set_tuning({"class_size": [1, 0, 0, 0, 0, 0, 0, 0],
                          "shape": [row(W) for W in range(17)],
                          "shape_compact": [row(W) for W in range(17)]})
@@ -124,7 +131,7 @@ class Context:
         # points of the 17..60-word groups (width > 16) time two shapes:
         # several words per lane (ms[0]) and one (ms[1])
         return [{"width": p.width, "dna": p.dna, "length": p.length, "pairs": p.pairs,
-                 "ms": ({_lib.SHAPES[k]: p.ms[k] for k in range(4) if p.ms[k] > 0} if p.width <= 16
+                 "ms": ({_lib.SHAPES[k]: p.ms[k] for k in range(5) if p.ms[k] > 0} if p.width <= 16
                         else {"multi": p.ms[0], "single": p.ms[1]})}
                 for p in pts[:min(n.value, cap)]]
 

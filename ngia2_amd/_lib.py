@@ -55,8 +55,8 @@ class BinStat(C.Structure):
 BIN_KINDS = {0: "zero", 1: "window", 2: "wave", 3: "plain", 4: "seg", 5: "seg2", 6: "seg3",
              7: "seg4"}
 
-SHAPE_WINDOW, SHAPE_SEG16, SHAPE_SEG32, SHAPE_WAVE = 0, 1, 2, 3
-SHAPES = ("window", "seg16", "seg32", "wave")
+SHAPE_WINDOW, SHAPE_SEG16, SHAPE_SEG32, SHAPE_WAVE, SHAPE_SEG8 = 0, 1, 2, 3, 4
+SHAPES = ("window", "seg16", "seg32", "wave", "seg8")
 TUNE_CLASSES = 8
 # groups of 17..60-word windows (17-21, 22-30, 32-45, 46-60) and their shapes
 LONG_GROUPS = 4
@@ -74,7 +74,7 @@ class Tuning(C.Structure):
 
 class TunePoint(C.Structure):
     _fields_ = [("width", C.c_uint32), ("dna", C.c_uint32), ("length", C.c_uint32),
-                ("pairs", C.c_uint32), ("ms", C.c_double * 4)]
+                ("pairs", C.c_uint32), ("ms", C.c_double * 5)]
 
 
 class Timing(C.Structure):

@@ -231,6 +231,7 @@ void enqueue_align(ngia_ctx* ctx, ngia_db* db, uint32_t identity, uint32_t mask,
   seg(kBinSeg16x4, fork ? ctx->side[2] : s);
   seg(kBinSeg16x3, fork ? ctx->side[3] : s);
   seg(kBinSeg8x3, fork ? ctx->side[1] : s);
+  seg(kBinSeg8x2, fork ? ctx->side[2] : s);
   seg(kBinSeg16x2, fork ? ctx->side[0] : s);
   seg(kBinSeg32, fork ? ctx->side[0] : s);
   seg(kBinSeg16, fork ? ctx->side[3] : s);

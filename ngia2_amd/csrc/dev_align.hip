@@ -858,7 +858,7 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
 
 // ------------------------------------------------ several pairs per wavefront
 // One pair per G-lane segment (64 / G pairs per wavefront), K words per lane
-// ((G, K) = (32, 1), (16, 1..4) or (8, 3)), for band windows of at most
+// ((G, K) = (32, 1), (16, 1..4) or (8, 2..3)), for band windows of at most
 // K (G - 1) words.  In the segment
 // of lanes [base, base + G), lane base + l holds the words B + K l + m,
 // m < K, of its pair's rep; B = K floor((cb - a) / K) trails the band bottom
@@ -894,8 +894,8 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
                                                   const uint32_t* __restrict__ count_ptr,
                                                   uint32_t* __restrict__ work,
                                                   uint32_t* __restrict__ score) {
-  static_assert((G == 32 && K == 1) || (G == 16 && K >= 1 && K <= 4) || (G == 8 && K == 3),
-                "segments of 32 lanes with one word, 16 with 1-4, 8 with 3");
+  static_assert((G == 32 && K == 1) || (G == 16 && K >= 1 && K <= 4) || (G == 8 && (K == 2 || K == 3)),
+                "segments of 32 lanes with one word, 16 with 1-4, 8 with 2-3");
   // floor(x / K) for either sign (x / K rounds toward zero)
   auto fdiv = [](int32_t x) { return (x >= 0 ? x : x - (K - 1)) / K; };
   // [K][16 codes][64 slots]; code 0 stays 0
@@ -1173,19 +1173,21 @@ int align_kernels_setup(size_t lds_per_block) {
   return kWaveKMax;
 }
 
-// The segmented kernel of (G, K): (16, 1..4), (32, 1) or (8, 3).
+// The segmented kernel of (G, K): (16, 1..4), (32, 1) or (8, 2..3).
 using SegKernel = void (*)(AlignArgs, const uint2*, const uint32_t*, const uint32_t*,
                            const uint32_t*, uint32_t*, uint32_t*);
 static int seg_index(int G, int K) {
   if (G == 16 && K >= 1 && K <= 4) return K - 1;
   if (G == 32 && K == 1) return 4;
   if (G == 8 && K == 3) return 5;
+  if (G == 8 && K == 2) return 6;
   throw HipError{NGIA_ERR_ARG,
                  "no segmented kernel for G=" + std::to_string(G) + ", K=" + std::to_string(K)};
 }
 static SegKernel seg_kernel(int G, int K) {
-  static const SegKernel k[6] = {k_align_seg<16, 1>, k_align_seg<16, 2>, k_align_seg<16, 3>,
-                                 k_align_seg<16, 4>, k_align_seg<32, 1>, k_align_seg<8, 3>};
+  static const SegKernel k[7] = {k_align_seg<16, 1>, k_align_seg<16, 2>, k_align_seg<16, 3>,
+                                 k_align_seg<16, 4>, k_align_seg<32, 1>, k_align_seg<8, 3>,
+                                 k_align_seg<8, 2>};
   return k[seg_index(G, K)];
 }
 
@@ -1193,7 +1195,7 @@ uint32_t seg_blocks_per_cu(int G, int K) {
   // Resident single-wave blocks per CU as registers and LDS allow (8 per
   // SIMD at 60 VGPRs and 4 KiB): twice the 16 that other bins get, which
   // took 25% off a bin of 2.4M pairs at windows of 17-31 words.
-  static int per_cu[6] = {0, 0, 0, 0, 0, 0};
+  static int per_cu[7] = {0, 0, 0, 0, 0, 0, 0};
   int& v = per_cu[seg_index(G, K)];
   if (!v) {
     int nb = 0;

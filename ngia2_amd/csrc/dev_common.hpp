@@ -195,7 +195,7 @@ void launch_align_wave(int K, const AlignArgs& a, const uint2* pairs, const uint
                        const uint32_t* off_ptr, const uint32_t* count_ptr, uint32_t* work, uint32_t* score,
                        uint32_t blocks, hipStream_t stream);
 // 64 / G pairs per wavefront, one per G-lane segment, K words per lane
-// ((G, K) = (16, 1..4), (32, 1) or (8, 3)), for band windows of at most
+// ((G, K) = (16, 1..4), (32, 1) or (8, 2..3)), for band windows of at most
 // K (G - 1) words; any alphabet (the 16-code table).
 size_t seg_lds_bytes();
 // Persistent blocks per CU for k_align_seg<G, K> (from the kernel's occupancy).

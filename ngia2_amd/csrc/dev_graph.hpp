@@ -9,8 +9,9 @@ namespace ngia {
 // kBinWave0 + log2(K) = k_align_wave<K>, kBinPlain = the plain kernel,
 // kBinSeg16 / kBinSeg32 = k_align_seg<16, 1> / <32, 1> (several pairs per
 // wavefront, one word per lane), kBinSeg16x2..x4 = k_align_seg<16, 2..4>
-// (four pairs per wavefront, 2-4 words per lane), kBinSeg8x3 =
-// k_align_seg<8, 3> (eight pairs per wavefront): band windows of
+// (four pairs per wavefront, 2-4 words per lane), kBinSeg8x3 / x2 =
+// k_align_seg<8, 3 / 2> (eight pairs per wavefront; x2 only for window bins
+// the tuning routes there): band windows of
 // kWinMax+1 .. 21 words go to kBinSeg8x3, of 22..30 to kBinSeg16x2, of 31
 // words to kBinSeg32, of 32..45 to kBinSeg16x3 and of 46..60 to
 // kBinSeg16x4.
@@ -23,7 +24,8 @@ constexpr int kBinSeg16x2 = kBinSeg32 + 1;
 constexpr int kBinSeg16x3 = kBinSeg16x2 + 1;
 constexpr int kBinSeg16x4 = kBinSeg16x3 + 1;
 constexpr int kBinSeg8x3 = kBinSeg16x4 + 1;
-constexpr int kBins = kBinSeg8x3 + 1;
+constexpr int kBinSeg8x2 = kBinSeg8x3 + 1;
+constexpr int kBins = kBinSeg8x2 + 1;
 static_assert(kBins <= 32, "bin masks are 32-bit");
 // A kBinSeg16x2 or kBinSeg8x3 bin of fewer pairs goes to kBinSeg32, a
 // kBinSeg16x3 or x4 bin to k_align_wave<1> (k_bin_offsets; the seg2_min
@@ -33,19 +35,24 @@ static_assert(kBins <= 32, "bin masks are 32-bit");
 // time.
 constexpr uint32_t kSeg2MinPairs = 32768;
 // The segmented bins and their (lanes per pair, words per lane).
-constexpr bool is_seg_bin(int b) { return b >= kBinSeg16 && b <= kBinSeg8x3; }
-constexpr int seg_lanes(int b) { return b == kBinSeg32 ? 32 : b == kBinSeg8x3 ? 8 : 16; }
+constexpr bool is_seg_bin(int b) { return b >= kBinSeg16 && b <= kBinSeg8x2; }
+constexpr int seg_lanes(int b) {
+  return b == kBinSeg32 ? 32 : (b == kBinSeg8x3 || b == kBinSeg8x2) ? 8 : 16;
+}
 constexpr int seg_words(int b) {
-  return b == kBinSeg8x3 ? 3 : b >= kBinSeg16x2 ? b - kBinSeg16x2 + 2 : 1;
+  return b == kBinSeg8x3 ? 3 : b == kBinSeg8x2 ? 2 : b >= kBinSeg16x2 ? b - kBinSeg16x2 + 2 : 1;
 }
 
 // Shapes a window bin (W = 2..kWinMax) can be aligned with, chosen per bin
 // size by the autotuner: one thread per pair (k_align_win<W>), 16 or 32 lanes
-// per pair (k_align_seg), or a wavefront per pair (k_align_wave<1>).
+// per pair (k_align_seg), a wavefront per pair (k_align_wave<1>), or 8 lanes
+// of two words per pair (k_align_seg<8, 2>).
 constexpr uint32_t kShapeWindow = NGIA_SHAPE_WINDOW;
 constexpr uint32_t kShapeSeg16 = NGIA_SHAPE_SEG16;
 constexpr uint32_t kShapeSeg32 = NGIA_SHAPE_SEG32;
 constexpr uint32_t kShapeWave = NGIA_SHAPE_WAVE;
+constexpr uint32_t kShapeSeg8 = NGIA_SHAPE_SEG8;
+constexpr uint32_t kShapes = kShapeSeg8 + 1;
 constexpr int kTuneClasses = NGIA_TUNE_CLASSES;
 // Window bin W holding c pairs is aligned with shape[W][k], k the last size
 // class with class_size[k] <= c (class 0 below class_size[0]).  A zeroed table
@@ -73,6 +80,8 @@ __host__ __device__ inline uint32_t route_window_bin(const RouteTable& rt, uint3
     case kShapeSeg16: return W <= 15 ? (uint32_t)kBinSeg16 : (uint32_t)kBinSeg32;
     case kShapeSeg32: return (uint32_t)kBinSeg32;
     case kShapeWave: return (uint32_t)kBinWave0;
+    case kShapeSeg8:
+      return W <= 14 ? (uint32_t)kBinSeg8x2 : W <= 15 ? (uint32_t)kBinSeg16 : (uint32_t)kBinSeg32;
     default: return W;
   }
 }

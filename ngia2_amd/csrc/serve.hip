@@ -483,7 +483,7 @@ void launch_batch(ngia_serve* sv, uint32_t k, const uint8_t* res, const uint64_t
                       std::min<uint32_t>(Q * 64, persistent_blocks(sv->ctx, wave_lds_bytes(1 << kk))), s);
   }
   for (const int bin :
-       {kBinSeg16x4, kBinSeg16x3, kBinSeg8x3, kBinSeg16x2, kBinSeg32, kBinSeg16}) {
+       {kBinSeg16x4, kBinSeg16x3, kBinSeg8x3, kBinSeg16x2, kBinSeg32, kBinSeg16, kBinSeg8x2}) {
     const int G = seg_lanes(bin), K = seg_words(bin);
     launch_align_seg(G, K, aa, ln.pairs.as<uint2>(), ln.list.as<uint32_t>(), &dc->bin_off[bin],
                      &dc->bin_cnt[bin], dc->queue[bin], ln.score.as<uint32_t>(),

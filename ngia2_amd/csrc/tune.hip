@@ -1,11 +1,13 @@
 // tune.hip -- the autotuner of the batched form's window bins.
 //
-// A window bin (band window W = 2..16 words) can be aligned in four shapes:
+// A window bin (band window W = 2..16 words) can be aligned in five shapes:
 // k_align_win<W>, one thread per pair (the most work per instruction, but a
-// lone pair's T x W word-characters run serially on one lane and a bin needs
-// hundreds of thousands of pairs to fill the device); k_align_seg<16> and
-// <32>, four and two pairs per wavefront with one word per lane; and
-// k_align_wave<1>, a wavefront per pair.  For each W the tuner times every
+// lone pair's T x W word-characters run serially on one lane, a bin needs
+// hundreds of thousands of pairs to fill the device, and the table of W x 4
+// KiB per wavefront leaves a CU two or three wavefronts at W >= 10);
+// k_align_seg<16> and <32>, four and two pairs per wavefront with one word
+// per lane; k_align_seg<8, 2>, eight pairs per wavefront with two words per
+// lane (W <= 14); and k_align_wave<1>, a wavefront per pair.  For each W the tuner times every
 // shape on synthetic pairs whose band window is exactly W (equal-length
 // random sequences) at bin sizes of 16 to 262,144 pairs, and installs the
 // fastest shape per (W, size class); k_bin_offsets then moves each window
@@ -49,7 +51,9 @@ float time_kernel(ngia_ctx* ctx, const std::function<void()>& launch, uint32_t* 
 }
 
 // Whether shape k can align a window of W words.
-bool shape_fits(uint32_t k, uint32_t W) { return k != kShapeSeg16 || W <= 15; }
+bool shape_fits(uint32_t k, uint32_t W) {
+  return k == kShapeSeg16 ? W <= 15 : k == kShapeSeg8 ? W <= 14 : true;
+}
 
 // The groups of 17..60-word windows: the window the tuner times each at, and
 // its two shapes as (lanes, words per lane), lanes 64 meaning k_align_wave<1>.
@@ -88,9 +92,9 @@ extern "C" int ngia_set_tuning(ngia_ctx* ctx, const ngia_tuning* t) {
     for (int k = 0; k < kTuneClasses; k++) {
       for (int tab = 0; tab < 2; tab++) {
         const uint32_t sh = tab ? t->shape_compact[W][k] : t->shape[W][k];
-        if (sh > kShapeWave) return fail(ctx, NGIA_ERR_ARG, "unknown shape");
+        if (sh >= kShapes) return fail(ctx, NGIA_ERR_ARG, "unknown shape");
         if (!shape_fits(sh, (uint32_t)W))
-          return fail(ctx, NGIA_ERR_ARG, "16-lane segments hold W <= 15");
+          return fail(ctx, NGIA_ERR_ARG, "16-lane segments hold W <= 15, 8-lane W <= 14");
         (tab ? rc : rt).shape[W][k] = (uint8_t)sh;
       }
     }
@@ -182,8 +186,8 @@ extern "C" int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* 
         const uint32_t zero = 0;
         h2d(off, &zero, 4, s);
         h2d(cnt, &n, 4, s);
-        double ms[4] = {0, 0, 0, 0};
-        for (uint32_t sh = kShapeWindow; sh <= kShapeWave; sh++) {
+        double ms[kShapes] = {0, 0, 0, 0, 0};
+        for (uint32_t sh = kShapeWindow; sh < kShapes; sh++) {
           if (!shape_fits(sh, (uint32_t)W)) continue;
           ms[sh] = time_kernel(ctx, [&] {
             if (sh == kShapeWindow)
@@ -192,15 +196,17 @@ extern "C" int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* 
             else if (sh == kShapeWave)
               launch_align_wave(1, aa, pairs.as<uint2>(), list.as<uint32_t>(), off, cnt, work,
                                 score.as<uint32_t>(), persistent_blocks(ctx, wave_lds_bytes(1)), s);
-            else
-              launch_align_seg(sh == kShapeSeg16 ? 16 : 32, 1, aa, pairs.as<uint2>(),
-                               list.as<uint32_t>(), off, cnt, work, score.as<uint32_t>(),
-                               ctx->prop.multiProcessorCount *
-                                   seg_blocks_per_cu(sh == kShapeSeg16 ? 16 : 32), s);
+            else {
+              const int G = sh == kShapeSeg8 ? 8 : sh == kShapeSeg16 ? 16 : 32;
+              const int K = sh == kShapeSeg8 ? 2 : 1;
+              launch_align_seg(G, K, aa, pairs.as<uint2>(), list.as<uint32_t>(), off, cnt, work,
+                               score.as<uint32_t>(),
+                               ctx->prop.multiProcessorCount * seg_blocks_per_cu(G, K), s);
+            }
           }, work);
         }
         uint32_t bestk = kShapeWindow;
-        for (uint32_t sh = kShapeSeg16; sh <= kShapeWave; sh++)
+        for (uint32_t sh = kShapeSeg16; sh < kShapes; sh++)
           if (ms[sh] > 0 && ms[sh] < ms[bestk]) bestk = sh;
         rt.shape[W][k] = (uint8_t)bestk;
         if (points && np < cap) {
@@ -209,7 +215,7 @@ extern "C" int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* 
           pt.dna = (uint32_t)dna;
           pt.length = L;
           pt.pairs = n;
-          for (int q = 0; q < 4; q++) pt.ms[q] = ms[q];
+          for (uint32_t q = 0; q < kShapes; q++) pt.ms[q] = ms[q];
         }
         np++;
       }
@@ -228,7 +234,7 @@ extern "C" int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* 
         const uint32_t zero = 0;
         h2d(off, &zero, 4, s);
         h2d(cnt, &n, 4, s);
-        double ms[4] = {0, 0, 0, 0};
+        double ms[kShapes] = {0, 0, 0, 0, 0};
         for (int which = 0; which < 2; which++) {
           ms[which] = time_kernel(ctx, [&] {
             if (which == 0)
@@ -253,7 +259,7 @@ extern "C" int ngia_autotune(ngia_ctx* ctx, uint32_t identity, ngia_tune_point* 
           pt.dna = 0;
           pt.length = L;
           pt.pairs = n;
-          for (int q = 0; q < 4; q++) pt.ms[q] = ms[q];
+          for (uint32_t q = 0; q < kShapes; q++) pt.ms[q] = ms[q];
         }
         np++;
       }

@@ -554,10 +554,10 @@ def test_eager_batched_and_bin_accounting(ctx, longdb, identity):
             assert b["ms"] > 0, b
 
 
-@pytest.mark.parametrize("shape", [ng.SHAPE_SEG16, ng.SHAPE_SEG32, ng.SHAPE_WAVE])
+@pytest.mark.parametrize("shape", [ng.SHAPE_SEG16, ng.SHAPE_SEG32, ng.SHAPE_WAVE, ng.SHAPE_SEG8])
 def test_routed_window_bins_score_alike(ctx, c1db, longdb, shape):
-    """Window bins routed to another shape (16 or 32 lanes per pair, or a
-    wavefront per pair) score every pair as the oracle does and as one thread
+    """Window bins routed to another shape (16 or 32 lanes per pair, 8 lanes
+    of two words, or a wavefront per pair) score every pair as the oracle does and as one thread
     per pair does, and the bin accounting follows the pairs: the shape's bins
     hold them and their tiles."""
     say("routed window bins, shape", ng.SHAPES[shape])
@@ -582,7 +582,7 @@ def test_routed_window_bins_score_alike(ctx, c1db, longdb, shape):
                 assert all(b["pairs"] == 0 for b in bins if b["kind"] == "window")
                 assert sum(b["ref_tiles"] for b in bins) == st["ref_tiles"]
                 if shape != ng.SHAPE_WAVE and win0:
-                    assert sum(b["pairs"] for b in bins if b["kind"] == "seg") >= win0
+                    assert sum(b["pairs"] for b in bins if b["kind"].startswith("seg")) >= win0
                 jobs = np.zeros((len(keys), 3), dtype=np.uint32)
                 jobs[:, 0] = keys >> np.uint64(32)
                 jobs[:, 1] = keys & np.uint64(0xFFFFFFFF)
@@ -722,7 +722,7 @@ def test_segmented_kernels_raw_lcs(ctx):
         del os.environ["NGIA_RAW_LCS"]
     seg_pairs = 0
     with raw_ctx:
-        for shape in (ng.SHAPE_SEG16, ng.SHAPE_SEG32):
+        for shape in (ng.SHAPE_SEG16, ng.SHAPE_SEG32, ng.SHAPE_SEG8):
             raw_ctx.route_all(shape)
             for identity in (30, 60, 90, 99):
                 say("segmented raw lcs, shape", ng.SHAPES[shape], "identity", identity)
@@ -804,7 +804,8 @@ def test_autotune_installs_fastest_shapes(ctx):
             want = ng.LONG_SINGLE if p["ms"]["single"] < p["ms"]["multi"] else ng.LONG_MULTI
             assert t["long"][g][k] == want, p
             continue
-        assert set(p["ms"]) == ({"window", "seg32", "wave"} | ({"seg16"} if p["width"] <= 15 else set()))
+        assert set(p["ms"]) == ({"window", "seg32", "wave"} | ({"seg16"} if p["width"] <= 15 else set())
+                                | ({"seg8"} if p["width"] <= 14 else set()))
         table = t["shape_compact"] if p["dna"] else t["shape"]
         best = min(p["ms"], key=lambda s_: (p["ms"][s_], ng.SHAPES.index(s_)))
         assert table[p["width"]][k] == ng.SHAPES.index(best), p
