@@ -813,6 +813,31 @@ def test_autotune_installs_fastest_shapes(ctx):
     assert ctx.tuning()["long"] == [[ng.LONG_DEFAULT] * 8] * 4
 
 
+def test_set_tuning_rejects_what_no_kernel_holds(ctx):
+    """ngia_set_tuning refuses shapes that cannot hold a window (16 lanes at
+    W = 16, 8 lanes of two words above 14 words), unknown shapes and unknown
+    long-window choices, and leaves the installed table as it was."""
+    base = ctx.tuning()
+    one = [1, 0, 0, 0, 0, 0, 0, 0]
+    bad = []
+    for W, sh in ((16, ng.SHAPE_SEG16), (15, ng.SHAPE_SEG8), (16, ng.SHAPE_SEG8), (5, 5)):
+        t = {"class_size": one, "shape": [[0] * 8 for _ in range(17)]}
+        t["shape"][W] = [sh] * 8
+        bad.append(t)
+    bad.append({"class_size": one, "long": [[3] * 8] * 4})
+    bad.append({"class_size": [16, 8, 0, 0, 0, 0, 0, 0]})  # sizes must ascend
+    for t in bad:
+        with pytest.raises(ng.NgiaError):
+            ctx.set_tuning(t)
+        assert ctx.tuning() == base
+    ok = {"class_size": one, "shape": [[0] * 8 for _ in range(17)], "long": [[2] * 8] * 4}
+    ok["shape"][14] = [ng.SHAPE_SEG8] * 8
+    ctx.set_tuning(ok)
+    t = ctx.tuning()
+    assert t["shape"][14] == [ng.SHAPE_SEG8] * 8 and t["long"] == [[2] * 8] * 4
+    ctx.set_tuning(base)
+
+
 def test_long_window_routing_table(ctx):
     """The tuned shapes of 17..60-word windows route their bins: with every
     group on several words per lane, then on one word per lane (or a
