@@ -857,6 +857,10 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
 }
 
 // ------------------------------------------------ several pairs per wavefront
+// The codes a segment without work reads: 32 zero bytes (code 0 matches
+// nothing, so its rows stay as they are).
+__device__ const uint4 kNoCodes[2] = {};
+
 // One pair per G-lane segment (64 / G pairs per wavefront), K words per lane
 // ((G, K) = (32, 1), (16, 1..4) or (8, 2..3)), for band windows of at most
 // K (G - 1) words.  In the segment
@@ -935,7 +939,10 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
       if (need) {
         need = false;
         active = false;
-        if (it < count) {
+        if (it >= count) {
+          tcq = kNoCodes;
+          c0 = c1 = make_uint4(0u, 0u, 0u, 0u);
+        } else {
           p = lst[it];
           const uint2 pr = pairs[p];
           rr = A.packed + A.word_off[pr.x];
@@ -996,11 +1003,13 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
     // This block's codes (held in registers by every lane of the segment);
     // the next block's are loaded now, to arrive during this one.
     const uint32_t cw[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-    if (active && cb + 1 < nb) {
-      c0 = tcq[2 * (cb + 1)];
-      c1 = tcq[2 * (cb + 1) + 1];
-    } else {
-      c0 = c1 = make_uint4(0u, 0u, 0u, 0u);
+    // Clamped to the pair's last block: a segment whose pair ends takes its
+    // next pair's codes at the claim, and one left without work reads the
+    // zero codes tcq then points at.
+    {
+      const uint32_t nx = active ? (cb + 1 < nb ? cb + 1 : nb - 1) : 0u;
+      c0 = tcq[2 * nx];
+      c1 = tcq[2 * nx + 1];
     }
     const uint32_t vs4 = vs * 4;
     // The masks of group g's four characters; those of group g + 1 are
