@@ -861,6 +861,14 @@ __global__ __launch_bounds__(64) void k_align_wave(AlignArgs A, const uint2* __r
 // nothing, so its rows stay as they are).
 __device__ const uint4 kNoCodes[2] = {};
 
+// Lane (id xor OFF) of the same 32-lane half (ds_swizzle's bit-mask mode:
+// and-mask 31, or-mask 0, xor-mask OFF).
+template <int OFF>
+__device__ inline uint32_t swizzle_xor(uint32_t v) {
+  static_assert(OFF >= 1 && OFF <= 16, "within 32 lanes");
+  return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (OFF << 10) | 0x1F);
+}
+
 // One pair per G-lane segment (64 / G pairs per wavefront), K words per lane
 // ((G, K) = (32, 1), (16, 1..4) or (8, 2..3)), for band windows of at most
 // K (G - 1) words.  In the segment
@@ -1087,7 +1095,14 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
       #pragma unroll
       for (int m = 0; m < K; m++) {
         if (l == 0) lcs += 32u - __popc(row[m]);
-        const uint32_t up = __shfl_down(row[m], 1);
+        // lane l + 1's word: within a 16-lane DPP row (row_shl:1) for
+        // segments of up to 16 lanes, where the row's last lane is a
+        // segment's top and takes a fresh word anyway
+        uint32_t up;
+        if constexpr (G <= 16)
+          up = (uint32_t)__builtin_amdgcn_mov_dpp((int)row[m], 0x101, 0xF, 0xF, true);
+        else
+          up = __shfl_down(row[m], 1);
         // lane G - 2 took the top lane's words: its last is a new word where
         // the sentinel was
         const bool fresh = l == (uint32_t)(G - 1) || (l == (uint32_t)(G - 2) && m == K - 1);
@@ -1109,7 +1124,13 @@ __global__ __launch_bounds__(64) void k_align_seg(AlignArgs A, const uint2* __re
       #pragma unroll
       for (int m = 0; m < K; m++)
         if (!sentinel(m)) part += 32u - __popc(row[m]);
-      for (int off = G / 2; off > 0; off >>= 1) part += __shfl_xor(part, off);
+      // butterfly sum over the segment: swizzles within 32 lanes (lane id
+      // xor off), no address arithmetic
+      if constexpr (G >= 32) part += swizzle_xor<16>(part);
+      if constexpr (G >= 16) part += swizzle_xor<8>(part);
+      part += swizzle_xor<4>(part);
+      part += swizzle_xor<2>(part);
+      part += swizzle_xor<1>(part);
       if (l == 0) score[p] = pair_value(A, part, R, T);
       if (A.tally && l == 0) ran += ref_tiles(R, T, A.thr, nb);
       active = false;
